@@ -1,0 +1,163 @@
+/*
+ * ivr_gpu.h -- C-ABI of the MI355X render path (libivr.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (0 = ok, negative = error; ivr_last_error() gives the message of the
+ * calling thread's last failure).  The caller owns host buffers; the library
+ * owns device memory behind the handles.  All GPU work of one context is
+ * ordered on that context's stream; ivr_frame_download() and ivr_ctx_sync()
+ * are the synchronisation points.  One context is used by one thread at a
+ * time.
+ *
+ * Reference interfaces replaced (file:line in the reference tree):
+ *   ivr_render             createlocalimage(int)            main.c:937-1026
+ *                          raycast/getsamplesonray/getlinearsample/
+ *                          getrayintersect                   ray.c:59-318
+ *                          transferfunc (CHEM, SEISMIC2, ...) transferfuncs.c:24-225
+ *                          compositetiles + RGBA8 quantiser  main.c:1203-1218, 1404
+ *   ivr_volume_upload      readvoldata/readfromfile + brick   main.c:629-706, 1220-1269,
+ *                          decomposition (getlocalbrick)     utils.c:332-388
+ *   ivr_tf_create          the colour LUT (lut, lutoffset)   main.c:1075-1200
+ *   ivr_state_orbit        the client's STATE producer       ivrclient.c:536-569
+ */
+#ifndef IVR_GPU_H
+#define IVR_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "ivr_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define IVR_OK 0
+#define IVR_ERR_ARG (-1)
+#define IVR_ERR_HIP (-2)
+#define IVR_ERR_NOMEM (-3)
+#define IVR_ERR_UNSUPPORTED (-4)
+#define IVR_ERR_BUDGET (-5)
+#define IVR_ERR_IO (-6)
+
+/* voxel storage types */
+#define IVR_F32 0
+#define IVR_F16 1
+
+/* render modes */
+#define IVR_MODE_COMPAT 0 /* exact ray.c semantics (dt = DELTASAMPLE t-units, 256-sample
+                             cap, reference sampler, back-to-front transfer function,
+                             projected-bbox pixel rectangle) */
+#define IVR_MODE_CONFIG 1 /* step in voxels, no cap, true trilinear, front-to-back
+                             compositing with early ray termination */
+
+/* compositing of bricks */
+#define IVR_FUSED 0     /* each ray walks all bricks front to back in one launch */
+#define IVR_SORT_LAST 1 /* one partial RGBA32F tile per brick, then an "over" pass
+                           in visibility order (the reference's sort-last scheme) */
+
+/* synthetic volumes */
+#define IVR_GEN_BLOBS 0       /* sum of Gaussian blobs, min-max normalised to [0,1] */
+#define IVR_GEN_NOISE_BLOBS 1 /* Perlin fBm + blobs, zeroed below a quantile, [0,1] */
+
+typedef struct ivr_ctx ivr_ctx;
+typedef struct ivr_volume ivr_volume;
+typedef struct ivr_tf ivr_tf;
+typedef struct ivr_frame ivr_frame;
+
+typedef struct ivr_gen_params {
+    int kind;          /* IVR_GEN_*                                          */
+    int nblobs;        /* Gaussian blobs: centres U[0.2,0.8]^3, sigma U[0.05,0.15],
+                          amplitude U[0.5,1]                                  */
+    int octaves;       /* fBm octaves                                        */
+    float lacunarity;  /* frequency ratio between octaves                    */
+    float gain;        /* amplitude ratio between octaves                    */
+    float base_freq;   /* octave-0 frequency, periods per volume edge        */
+    float empty_frac;  /* quantile below which voxels are set to 0           */
+    float value_scale; /* final values are multiplied by this (1 or 255)      */
+    uint64_t seed;
+} ivr_gen_params;
+
+typedef struct ivr_frame_desc {
+    ivr_state state;   /* camera, view volume, cut planes, alpha, width, height */
+    int mode;          /* IVR_MODE_*                                          */
+    int xfer;          /* compat: transfer function id (transferfuncs.h:8-13) */
+    float step_vox;    /* config: sample spacing in voxels                    */
+    float ert;         /* config: stop once accumulated alpha >= ert (>1: off) */
+    float lutscale;    /* LUT index = (int)(s * lutscale) - lutoffset          */
+    int lutoffset;
+    int maxlutindx;    /* SEISMIC2 upper clamp                                 */
+    int skip;          /* config: macrocell empty-space skipping on/off        */
+    int composite;     /* IVR_FUSED or IVR_SORT_LAST                           */
+} ivr_frame_desc;
+
+typedef struct ivr_frame_stats {
+    unsigned long long samples; /* samples classified and composited           */
+    unsigned long long visited; /* samples whose voxels were fetched (after skip) */
+    float render_ms;            /* device time of the last render (events)      */
+    float composite_ms;         /* device time of the sort-last over pass        */
+} ivr_frame_stats;
+
+/* ---- context ---- */
+int ivr_ctx_create(int device, ivr_ctx **out);
+int ivr_ctx_destroy(ivr_ctx *ctx);
+int ivr_ctx_sync(ivr_ctx *ctx);
+/* record event `slot` (0..15) on the context stream; elapsed ms between two */
+int ivr_ctx_record(ivr_ctx *ctx, int slot);
+int ivr_ctx_elapsed(ivr_ctx *ctx, int slot0, int slot1, float *ms);
+/* bytes of device memory held by the context's objects */
+unsigned long long ivr_ctx_device_bytes(const ivr_ctx *ctx);
+const char *ivr_last_error(void);
+
+/* ---- volumes: nx*ny*nz scalars, stored as bricks of brick^3 voxels plus a
+ * one-voxel ghost layer on every side, bricks in Morton order ---- */
+int ivr_volume_create(ivr_ctx *ctx, int nx, int ny, int nz, int dtype, int brick,
+                      ivr_volume **out);
+int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *vol, const float *host_xfastest);
+int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *vol, const ivr_gen_params *gp);
+int ivr_volume_download(ivr_ctx *ctx, const ivr_volume *vol, float *host_xfastest);
+int ivr_volume_destroy(ivr_ctx *ctx, ivr_volume *vol);
+/* dims[3], nbricks[3], brick edge, dtype, device bytes */
+int ivr_volume_info(const ivr_volume *vol, int dims[3], int nbricks[3], int *brick,
+                    int *dtype, unsigned long long *bytes);
+
+/* ---- transfer function LUT (n RGBA entries, n <= 4096) ---- */
+int ivr_tf_create(ivr_ctx *ctx, const float *rgba, int n, ivr_tf **out);
+int ivr_tf_update(ivr_ctx *ctx, ivr_tf *tf, const float *rgba, int n);
+int ivr_tf_destroy(ivr_ctx *ctx, ivr_tf *tf);
+
+/* ---- frames: device RGBA8 image (+ optional RGBA32F), row 0 = bottom ---- */
+int ivr_frame_create(ivr_ctx *ctx, int width, int height, int keep_rgba32, ivr_frame **out);
+int ivr_frame_destroy(ivr_ctx *ctx, ivr_frame *fr);
+/* waits for the frame's work; either pointer may be NULL */
+int ivr_frame_download(ivr_ctx *ctx, ivr_frame *fr, unsigned char *rgba8, float *rgba32);
+int ivr_frame_stats_get(ivr_ctx *ctx, ivr_frame *fr, ivr_frame_stats *st);
+
+/* ---- render: enqueue one frame on the context stream ---- */
+int ivr_render(ivr_ctx *ctx, const ivr_volume *vol, const ivr_tf *tf,
+               const ivr_frame_desc *desc, ivr_frame *fr);
+
+/* ---- host-side helpers (no GPU) ---- */
+/* Defaults for a config-mode frame of the given state. */
+void ivr_desc_config(ivr_frame_desc *d);
+/* Orbit camera: STATE of a client looking at the origin from distance 5 after
+ * rotating the scene by azimuth/elevation drawn from `seed` (radians). */
+void ivr_state_orbit(uint64_t seed, int width, int height, ivr_state *st,
+                     float *az_out, float *el_out);
+void ivr_state_from_angles(float az, float el, int width, int height, ivr_state *st);
+/* 256-entry piecewise-linear RGBA LUT from 5 control points drawn from `seed`;
+ * alpha is 0 at value 0 and U[0, alpha_max] elsewhere.  When jitter > 0 the
+ * control points are perturbed by U[-jitter, jitter] drawn from jitter_seed. */
+void ivr_tf_build(uint64_t seed, float alpha_max, float jitter, uint64_t jitter_seed,
+                  float *lut256);
+/* CPU evaluation of the synthetic generator, bit-identical to the device one
+ * (same code, same operation order); out is x-fastest. */
+int ivr_generate_host(const ivr_gen_params *gp, int nx, int ny, int nz, float *out);
+/* deterministic PRNG used by every generator: SplitMix64-seeded PCG32 */
+uint32_t ivr_rng_u32(uint64_t *state);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+/*
+ * ref_harness.c -- TEST INFRASTRUCTURE ONLY (never linked into the product).
+ *
+ * Drives the reference renderer, compiled from its own C sources under
+ * /root/reference by oracle/Makefile, headless.  The reference has no test
+ * driver: its main() only initialises unless built with MPI or LOCAL
+ * (main.c:171-258), so this file replays the per-frame sequence of the
+ * reference server loop (main.c:226-233) and exposes the per-ray, per-sample,
+ * transfer-function and colormap entry points through plain C functions that
+ * the golden-vector script (tests/golden/make_golden.py) calls with ctypes.
+ *
+ * Built only into oracle/_ref/libivrref.so.  Nothing here is shipped.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "types.h"
+#include "connection.h"
+#include "mathutil.h"
+#include "transferfuncs.h"
+#include "colormap.h"
+
+/* reference globals (main.c:63-105, ray.c:38) */
+extern VOLUME volume;
+extern AABB aabb;
+extern STATE mystate;
+extern color4f *lut;
+extern color4f *colorbuf;
+extern int lutoffset;
+extern int maxlutindx;
+extern int winwidth, winheight;
+extern float pzero[6];
+extern float dconst[6];
+
+/* reference functions (main.c, ray.c, mathutil.c) */
+int init(CMDARGS, int);
+void initcolbufs(int);
+void createlocalimage(int);
+void setplaneconst(float *, float *);
+int compositetiles(float *, float *, int, int, int);
+void raycast(ray3f, color4f *);
+int getsamplesonray(ray3f, int, float *);
+boolean_t getrayintersect(ray3f, float *, float *);
+float getlinearsample(float, float, float);
+void initsamplefunc(int);
+
+static color4f *g_user_lut = NULL;
+
+/* init(): volume file of nx*ny*nz little-endian float32, x fastest. */
+int ivrref_init(const char *volfile, int nx, int ny, int nz, int cmap, int xfer,
+                int tile)
+{
+    CMDARGS a;
+    memset(&a, 0, sizeof(a));
+    strncpy(a.filename, volfile, sizeof(a.filename) - 1);
+    a.width = nx;
+    a.height = ny;
+    a.depth = nz;
+    a.dataendian = 1; /* 1 = little endian, sysutil.h:110-115 */
+    a.colormap = cmap;
+    a.xfer = xfer;
+    a.procdims[0] = a.procdims[1] = a.procdims[2] = 1;
+    a.tilewid = a.tilehgt = tile;
+    return init(a, 0);
+}
+
+/* Replace the histogram LUT by a caller LUT (n entries of RGBA). */
+void ivrref_set_lut(const float *rgba, int n, int offset, int maxidx)
+{
+    free(g_user_lut);
+    g_user_lut = (color4f *)malloc(sizeof(color4f) * (size_t)n);
+    memcpy(g_user_lut, rgba, sizeof(color4f) * (size_t)n);
+    lut = g_user_lut;
+    lutoffset = offset;
+    maxlutindx = maxidx;
+}
+
+int ivrref_lutoffset(void) { return lutoffset; }
+void ivrref_get_lut(float *out, int n) { memcpy(out, lut, sizeof(color4f) * (size_t)n); }
+
+void ivrref_box(float out[6])
+{
+    out[0] = aabb.min.x; out[1] = aabb.min.y; out[2] = aabb.min.z;
+    out[3] = aabb.max.x; out[4] = aabb.max.y; out[5] = aabb.max.z;
+}
+
+/* One frame as the server loop renders it (main.c:226-233, 239).  `out`
+ * receives W*H RGBA32F in colorbuf order (row 0 = bottom).  rect receives
+ * the cast pixel rectangle istrt, istop, jstrt, jstop (main.c:959-962). */
+int ivrref_render(const float mvm_cm[16], const float pm_cm[16], const float vv[4],
+                  const float newconst[6], float alpha, int W, int H,
+                  float *out, int rect[4])
+{
+    float planeconst[6];
+    float nc[6];
+    float m[16], p[16], v4[4];
+    memcpy(m, mvm_cm, sizeof m);
+    memcpy(p, pm_cm, sizeof p);
+    memcpy(v4, vv, sizeof v4);
+    memcpy(nc, newconst, sizeof nc);
+    memcpy(planeconst, pzero, sizeof planeconst);
+    winwidth = W;
+    winheight = H;
+    mystate.alpha = alpha;
+    setviewport(&viewport, 0, 0, W, H);
+    setplaneconst(planeconst, nc);
+    setmvm(mvm, m);
+    setpm(pm, p);
+    setfrustum(&viewvolume, v4);
+    updatevolbbox();
+    initcolbufs(0);
+    /* createlocalimage may write one row past the viewport when the box
+     * touches the top edge (main.c:960); give it room. */
+    free(colorbuf);
+    colorbuf = (color4f *)calloc((size_t)W * (size_t)(H + 2), sizeof(color4f));
+    createlocalimage(0);
+    memcpy(out, colorbuf, sizeof(color4f) * (size_t)W * (size_t)H);
+    rect[0] = (int)(aabb.wincoord.y - 0.5f);
+    rect[1] = (int)(aabb.wincoord.y + aabb.wincoord.h + 1.0f);
+    rect[2] = (int)(aabb.wincoord.x - 0.5f);
+    rect[3] = (int)(aabb.wincoord.x + aabb.wincoord.w + 1.0f);
+    return 0;
+}
+
+static ray3f mkray(const float r[6])
+{
+    ray3f ray;
+    ray.orig.x = r[0]; ray.orig.y = r[1]; ray.orig.z = r[2];
+    ray.dir.x = r[3]; ray.dir.y = r[4]; ray.dir.z = r[5];
+    return ray;
+}
+
+float ivrref_getlinearsample(float x, float y, float z) { return getlinearsample(x, y, z); }
+
+int ivrref_getrayintersect(const float r[6], float *tin, float *tout)
+{
+    return getrayintersect(mkray(r), tin, tout) ? 1 : 0;
+}
+
+int ivrref_getsamplesonray(const float r[6], int maxs, float *samples)
+{
+    initsamplefunc(1);
+    return getsamplesonray(mkray(r), maxs, samples);
+}
+
+void ivrref_raycast(const float r[6], float alpha, float out[4])
+{
+    color4f c;
+    mystate.alpha = alpha;
+    raycast(mkray(r), &c);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+}
+
+/* transfer function `xfer` (transferfuncs.h:8-13) over a caller sample
+ * array with the current lut/lutoffset/maxlutindx. */
+void ivrref_transferfunc(int xfer, float *samples, int ns, float alpha, float out[4])
+{
+    color4f c;
+    c.r = c.g = c.b = c.a = 0.0f;
+    inittransferfunc(xfer);
+    c = (*transferfunc)(lut, c, samples, ns, alpha);
+    out[0] = c.r; out[1] = c.g; out[2] = c.b; out[3] = c.a;
+}
+
+/* colormap `id` (colormap.h:9-14) into out[hsize*4]; hist is all ones. */
+void ivrref_colormap(int id, int hsize, float hmin, float hmax, float *out)
+{
+    int *hist = (int *)malloc(sizeof(int) * (size_t)hsize);
+    color4f *l = (color4f *)calloc((size_t)hsize, sizeof(color4f));
+    for (int i = 0; i < hsize; i++) hist[i] = 1;
+    initcolormap(id);
+    (*colormap)(l, hist, hsize, hmin, hmax);
+    memcpy(out, l, sizeof(color4f) * (size_t)hsize);
+    free(hist);
+    free(l);
+}
+
+int ivrref_compositetiles(float *dst, float *tiles, int num, int w, int h)
+{
+    return compositetiles(dst, tiles, num, w, h);
+}
