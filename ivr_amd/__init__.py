@@ -1,0 +1,302 @@
+"""Python mirror of the libivr.so C-ABI (include/ivr_gpu.h).
+
+The reference drives its renderer from C through globals and a per-frame
+STATE (connection.h:9-18); this module exposes the same render path to
+Python for tests, the benchmark and the steering server: a Context owns the
+HIP stream, Volume/TF/Frame wrap the device objects, and Context.render
+enqueues one frame.  Every call goes through the compiled library; there is
+no Python or CPU fallback, and importing this module fails loudly when
+libivr.so is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libivr.so")
+
+IVR_F32, IVR_F16 = 0, 1
+MODE_COMPAT, MODE_CONFIG = 0, 1
+FUSED, SORT_LAST = 0, 1
+GEN_BLOBS, GEN_NOISE_BLOBS = 0, 1
+
+# reference transfer function ids (transferfuncs.h:8-13)
+SEISMIC, MEDXFER, SEISMIC2, CHEM, BLEND, RGBBLEND = 0, 1, 2, 3, 4, 5
+
+
+class IvrError(RuntimeError):
+    pass
+
+
+class State(C.Structure):
+    """STATE (connection.h:9-18), 184 bytes."""
+    _fields_ = [("finish", C.c_int), ("winwidth", C.c_int), ("winheight", C.c_int),
+                ("alpha", C.c_float), ("newconst", C.c_float * 6), ("vv", C.c_float * 4),
+                ("mvm", C.c_float * 16), ("pm", C.c_float * 16)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int), ("nblobs", C.c_int), ("octaves", C.c_int),
+                ("lacunarity", C.c_float), ("gain", C.c_float), ("base_freq", C.c_float),
+                ("empty_frac", C.c_float), ("value_scale", C.c_float), ("seed", C.c_uint64)]
+
+
+class FrameDesc(C.Structure):
+    _fields_ = [("state", State), ("mode", C.c_int), ("xfer", C.c_int), ("step_vox", C.c_float),
+                ("ert", C.c_float), ("lutscale", C.c_float), ("lutoffset", C.c_int),
+                ("maxlutindx", C.c_int), ("skip", C.c_int), ("composite", C.c_int)]
+
+
+class FrameStats(C.Structure):
+    _fields_ = [("samples", C.c_ulonglong), ("visited", C.c_ulonglong),
+                ("render_ms", C.c_float), ("composite_ms", C.c_float)]
+
+
+assert C.sizeof(State) == 184
+assert C.sizeof(GenParams) == 40
+assert C.sizeof(FrameDesc) == 220
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libivr.so not built at %s: run __graft_entry__.build() "
+                          "(python ivr_amd/build.py)" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    sig = {
+        "ivr_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+        "ivr_ctx_destroy": (C.c_int, [P]),
+        "ivr_ctx_sync": (C.c_int, [P]),
+        "ivr_ctx_record": (C.c_int, [P, C.c_int]),
+        "ivr_ctx_elapsed": (C.c_int, [P, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+        "ivr_ctx_device_bytes": (C.c_ulonglong, [P]),
+        "ivr_last_error": (C.c_char_p, []),
+        "ivr_volume_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+        "ivr_volume_upload": (C.c_int, [P, P, P]),
+        "ivr_volume_generate": (C.c_int, [P, P, C.POINTER(GenParams)]),
+        "ivr_volume_download": (C.c_int, [P, P, P]),
+        "ivr_volume_destroy": (C.c_int, [P, P]),
+        "ivr_volume_info": (C.c_int, [P, P, P, P, P, P]),
+        "ivr_tf_create": (C.c_int, [P, P, C.c_int, C.POINTER(P)]),
+        "ivr_tf_update": (C.c_int, [P, P, P, C.c_int]),
+        "ivr_tf_destroy": (C.c_int, [P, P]),
+        "ivr_frame_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+        "ivr_frame_destroy": (C.c_int, [P, P]),
+        "ivr_frame_download": (C.c_int, [P, P, P, P]),
+        "ivr_frame_stats_get": (C.c_int, [P, P, C.POINTER(FrameStats)]),
+        "ivr_render": (C.c_int, [P, P, P, C.POINTER(FrameDesc), P]),
+        "ivr_desc_config": (None, [C.POINTER(FrameDesc)]),
+        "ivr_state_orbit": (None, [C.c_uint64, C.c_int, C.c_int, C.POINTER(State),
+                                   C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "ivr_state_from_angles": (None, [C.c_float, C.c_float, C.c_int, C.c_int, C.POINTER(State)]),
+        "ivr_tf_build": (None, [C.c_uint64, C.c_float, C.c_float, C.c_uint64, P]),
+        "ivr_generate_host": (C.c_int, [C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, P]),
+        "ivr_rng_u32": (C.c_uint32, [C.POINTER(C.c_uint64)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def check(rc):
+    if rc != 0:
+        raise IvrError("libivr error %d: %s" % (rc, lib.ivr_last_error().decode()))
+    return rc
+
+
+# ---------------------------------------------------------------- host helpers
+
+def orbit_state(seed, width, height):
+    st = State()
+    az, el = C.c_float(), C.c_float()
+    lib.ivr_state_orbit(seed, width, height, C.byref(st), C.byref(az), C.byref(el))
+    return st
+
+
+def state_from_angles(az, el, width, height):
+    st = State()
+    lib.ivr_state_from_angles(az, el, width, height, C.byref(st))
+    return st
+
+
+def tf_build(seed, alpha_max=0.05, jitter=0.0, jitter_seed=0):
+    lut = np.zeros((256, 4), np.float32)
+    lib.ivr_tf_build(seed, alpha_max, jitter, jitter_seed, _ptr(lut))
+    return lut
+
+
+def gen_params(kind, seed, nblobs, octaves=4, lacunarity=2.0, gain=0.5, base_freq=4.0,
+               empty_frac=0.0, value_scale=1.0):
+    return GenParams(kind=kind, nblobs=nblobs, octaves=octaves, lacunarity=lacunarity, gain=gain,
+                     base_freq=base_freq, empty_frac=empty_frac, value_scale=value_scale, seed=seed)
+
+
+def generate_host(gp, nx, ny, nz):
+    out = np.empty((nz, ny, nx), np.float32)
+    check(lib.ivr_generate_host(C.byref(gp), nx, ny, nz, _ptr(out)))
+    return out
+
+
+def config_desc(state, skip=False, composite=FUSED, ert=0.99, step_vox=0.5, lutscale=255.0):
+    d = FrameDesc()
+    lib.ivr_desc_config(C.byref(d))
+    d.state = state
+    d.skip = int(skip)
+    d.composite = composite
+    d.ert = ert
+    d.step_vox = step_vox
+    d.lutscale = lutscale
+    return d
+
+
+def compat_desc(state, xfer=CHEM, lutoffset=0, maxlutindx=0):
+    d = FrameDesc()
+    lib.ivr_desc_config(C.byref(d))
+    d.state = state
+    d.mode = MODE_COMPAT
+    d.xfer = xfer
+    d.lutscale = 1.0
+    d.lutoffset = lutoffset
+    d.maxlutindx = maxlutindx
+    d.ert = 2.0
+    return d
+
+
+# ---------------------------------------------------------------- device objects
+
+class Context:
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib.ivr_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.ivr_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def sync(self):
+        check(lib.ivr_ctx_sync(self.h))
+
+    def record(self, slot):
+        check(lib.ivr_ctx_record(self.h, slot))
+
+    def elapsed(self, s0, s1):
+        ms = C.c_float()
+        check(lib.ivr_ctx_elapsed(self.h, s0, s1, C.byref(ms)))
+        return ms.value
+
+    def device_bytes(self):
+        return lib.ivr_ctx_device_bytes(self.h)
+
+    def volume(self, nx, ny, nz, dtype=IVR_F32, brick=256):
+        return Volume(self, nx, ny, nz, dtype, brick)
+
+    def tf(self, lut):
+        return TF(self, lut)
+
+    def frame(self, width, height, keep_rgba32=False):
+        return Frame(self, width, height, keep_rgba32)
+
+    def render(self, vol, tf, desc, frame):
+        check(lib.ivr_render(self.h, vol.h, tf.h, C.byref(desc), frame.h))
+
+
+class Volume:
+    def __init__(self, ctx, nx, ny, nz, dtype, brick):
+        self.ctx = ctx
+        self.shape = (nz, ny, nx)
+        h = C.c_void_p()
+        check(lib.ivr_volume_create(ctx.h, nx, ny, nz, dtype, brick, C.byref(h)))
+        self.h = h
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        check(lib.ivr_volume_upload(self.ctx.h, self.h, _ptr(a)))
+
+    def generate(self, gp):
+        check(lib.ivr_volume_generate(self.ctx.h, self.h, C.byref(gp)))
+
+    def download(self):
+        out = np.empty(self.shape, np.float32)
+        check(lib.ivr_volume_download(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def info(self):
+        dims = (C.c_int * 3)()
+        nb = (C.c_int * 3)()
+        brick, dtype = C.c_int(), C.c_int()
+        nbytes = C.c_ulonglong()
+        check(lib.ivr_volume_info(self.h, dims, nb, C.byref(brick), C.byref(dtype), C.byref(nbytes)))
+        return dict(dims=tuple(dims), nbricks=tuple(nb), brick=brick.value, dtype=dtype.value,
+                    bytes=nbytes.value)
+
+    def close(self):
+        if self.h:
+            lib.ivr_volume_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class TF:
+    def __init__(self, ctx, lut):
+        self.ctx = ctx
+        a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+        h = C.c_void_p()
+        check(lib.ivr_tf_create(ctx.h, _ptr(a), a.shape[0], C.byref(h)))
+        self.h = h
+        self.n = a.shape[0]
+
+    def update(self, lut):
+        a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
+        check(lib.ivr_tf_update(self.ctx.h, self.h, _ptr(a), a.shape[0]))
+        self.n = a.shape[0]
+
+    def close(self):
+        if self.h:
+            lib.ivr_tf_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+class Frame:
+    def __init__(self, ctx, width, height, keep_rgba32):
+        self.ctx = ctx
+        self.width, self.height = width, height
+        self.keep32 = keep_rgba32
+        h = C.c_void_p()
+        check(lib.ivr_frame_create(ctx.h, width, height, int(keep_rgba32), C.byref(h)))
+        self.h = h
+
+    def download(self, rgba32=False):
+        img8 = np.empty((self.height, self.width, 4), np.uint8)
+        img32 = np.empty((self.height, self.width, 4), np.float32) if rgba32 else None
+        check(lib.ivr_frame_download(self.ctx.h, self.h, _ptr(img8),
+                                     _ptr(img32) if rgba32 else None))
+        return (img8, img32) if rgba32 else img8
+
+    def stats(self):
+        st = FrameStats()
+        check(lib.ivr_frame_stats_get(self.ctx.h, self.h, C.byref(st)))
+        return dict(samples=st.samples, visited=st.visited, render_ms=st.render_ms,
+                    composite_ms=st.composite_ms)
+
+    def close(self):
+        if self.h:
+            lib.ivr_frame_destroy(self.ctx.h, self.h)
+            self.h = None

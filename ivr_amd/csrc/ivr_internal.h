@@ -16,6 +16,7 @@
 #define IVR_CELL_SHIFT 4 /* macrocells of 16^3 voxels */
 #define IVR_MAX_LUT 4096
 #define IVR_MAX_BRICKS 4096
+#define IVR_MAX_STEPS (1 << 22) /* per-ray sample bound */
 #define IVR_BLOCK 256 /* 4 wavefronts = 2x2 tiles of 8x8 pixels */
 
 int ivr_fail(int code, const char *fmt, ...);

@@ -164,7 +164,9 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
     tin = fmaxf(tin, 0.0f);
     const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     const float dt = (a.vox * a.step_vox) / dlen;
-    const int n = (int)(1.0f + (tout - tin) / dt + 0.5f);
+    const float nf = 1.0f + (tout - tin) / dt + 0.5f;
+    /* guard for degenerate directions; never reached by a finite ray */
+    const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
     /* voxel-space ray for macrocell exits: g(t) ~ go + gd t */
     float go[3], rgd[3], gd[3];
     if (SKIP) {
@@ -431,7 +433,8 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
             tin = fmaxf(tin, 0.0f);
             const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
             const float dt = (a.vox * a.step_vox) / dlen;
-            const int n = (int)(1.0f + (tout - tin) / dt + 0.5f);
+            const float nf = 1.0f + (tout - tin) / dt + 0.5f;
+            const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
             /* k range of the brick's ownership box (open towards the volume
              * faces, where clamped samples belong to the boundary brick);
              * grazing rays scan the whole grid */

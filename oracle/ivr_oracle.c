@@ -609,6 +609,25 @@ static col4 ray_compat(const rctx *c, const float o[3], const float d[3], float 
     return tf_apply(c->p->xfer, c->lut, c->p->lutoffset, c->p->maxlutindx, buf, i, c->alpha);
 }
 
+int ivro_samplesonray_compat(const float *vol, int nx, int ny, int nz, const float newconst[6],
+                             const float ray[6], int maxs, float *samples)
+{
+    float bmin[3], bmax[3], dc[6], tin, tout;
+    ivro_box(nx, ny, nz, bmin, bmax);
+    plane_consts(bmin, bmax, newconst, dc);
+    if (!slab(dc, ray, ray + 3, &tin, &tout)) return -1;
+    const float dt = 0.001f;
+    int iend = 1.0f + (tout - tin) / dt + 0.5f;
+    iend = (iend < maxs) ? iend : maxs;
+    int i;
+    for (i = 0; i < iend; i++) {
+        float t = tin + (i * dt);
+        samples[i] = ivro_sample_ref(vol, nx, ny, nz, bmin, bmax, ray[3] * t + ray[0],
+                                     ray[4] * t + ray[1], ray[5] * t + ray[2]);
+    }
+    return i;
+}
+
 /* Config mode: global sample grid t_k = tin + k*dt, dt = step voxels / |dir|,
  * n = (int)(1 + (tout - tin)/dt + 0.5) (the count rule of ray.c:253),
  * front-to-back C += (1-A) a c, A += (1-A) a, stop at A >= ert.  In tile mode

@@ -76,6 +76,9 @@ void ivrref_set_lut(const float *rgba, int n, int offset, int maxidx)
     maxlutindx = maxidx;
 }
 
+/* select transfer function `xfer` for following renders (transferfuncs.c:201-225) */
+void ivrref_set_xfer(int xfer) { inittransferfunc(xfer); }
+
 int ivrref_lutoffset(void) { return lutoffset; }
 void ivrref_get_lut(float *out, int n) { memcpy(out, lut, sizeof(color4f) * (size_t)n); }
 

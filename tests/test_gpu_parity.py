@@ -1,0 +1,225 @@
+"""GPU parity of the HIP render path (libivr.so through its C-ABI) against
+the reference renderer's golden vectors and the CPU oracle.
+
+Bar: compat mode bit-exact with the compiled reference's RGBA32F images;
+config mode within 1/255 per RGBA8 channel of the fp32 oracle (it is
+expected to be bit-exact: both compute in the same float order) with RMSE
+and the FP64 oracle error reported; invariances (bricking, macrocell
+skipping, determinism) bit-exact.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import ivr_amd as ivr
+import oracle
+from ivr_amd import configs
+
+pytestmark = pytest.mark.gpu
+G = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+META = json.load(open(os.path.join(G, "golden_meta.json")))
+
+
+def _render(ctx, vol, lut, desc, W, H):
+    tf = ctx.tf(lut)
+    fr = ctx.frame(W, H, keep_rgba32=True)
+    ctx.render(vol, tf, desc, fr)
+    img8, img32 = fr.download(rgba32=True)
+    st = fr.stats()
+    fr.close()
+    tf.close()
+    return img8, img32, st
+
+
+def _report(tag, img8, ref32, img32=None):
+    ref8 = oracle.quantize(ref32)
+    d8 = np.abs(img8.astype(int) - ref8.astype(int))
+    rmse = float(np.sqrt(np.mean((img8.astype(float) - ref8.astype(float)) ** 2)))
+    same = None if img32 is None else float(np.mean(img32.view(np.uint32) == ref32.view(np.uint32)))
+    print("%s: max|d8|=%d rmse8=%.4f bitwise-identical=%s" % (tag, d8.max(), rmse, same), flush=True)
+    return d8.max(), rmse, same
+
+
+@pytest.fixture(scope="module")
+def c1_host():
+    gp = configs.gen_params(1)
+    return ivr.generate_host(gp, 64, 64, 64)
+
+
+def test_compat_matches_compiled_reference(ctx):
+    """GPU compat mode == the reference's createlocalimage, bit for bit."""
+    gp = configs.gen_params(1)
+    gp.value_scale = 255.0
+    host = ivr.generate_host(gp, 64, 64, 64)
+    assert hashlib.sha256(host.tobytes()).hexdigest() == META["sha256"]
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    vol.upload(host)
+    z = np.load(os.path.join(G, "compat_images.npz"))
+    luts = {"tf5": z["lut_tf5"], "medical": z["lut_medical"], "jet": z["lut_jet"]}
+    W, H = META["width"], META["height"]
+    for k, case in enumerate(META["compat_cases"]):
+        print("compat case", k, case, flush=True)
+        st = ivr.state_from_angles(case["az"], case["el"], W, H)
+        st.alpha = case["alpha"]
+        d = ivr.compat_desc(st, xfer=case["xfer"], maxlutindx=case["maxlutindx"])
+        img8, img32, stats = _render(ctx, vol, luts[case["lut"]], d, W, H)
+        ref = z["images"][k]
+        assert np.array_equal(img32.view(np.uint32), ref.view(np.uint32)), case
+        assert np.array_equal(img8, oracle.quantize(ref))
+    vol.close()
+
+
+def test_generator_gpu_equals_host(ctx):
+    for (kind_cfg, n, brick) in ((1, 64, 64), (2, 96, 32)):
+        gp = configs.gen_params(kind_cfg)
+        host = ivr.generate_host(gp, n, n, n)
+        vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=brick)
+        vol.generate(gp)
+        dev = vol.download()
+        vol.close()
+        assert np.array_equal(dev.view(np.uint32), host.view(np.uint32)), kind_cfg
+        if kind_cfg == 2:
+            zf = float(np.mean(host == 0.0))
+            assert abs(zf - 0.4) < 0.01, zf
+
+
+def test_config_c1_matches_oracle(ctx, c1_host):
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    vol.upload(c1_host)
+    lut = configs.tf_lut(1)
+    st = configs.state(1)
+    d = ivr.config_desc(st)
+    img8, img32, stats = _render(ctx, vol, lut, d, 256, 256)
+    ref32, s_ref = oracle.render(c1_host, oracle.view_of(st), lut, oracle.params())
+    ref64, _ = oracle.render(c1_host, oracle.view_of(st), lut, oracle.params(), f64=True)
+    m, rmse, same = _report("C1 config vs fp32 oracle", img8, ref32, img32)
+    e64 = float(np.abs(img32.astype(np.float64) - ref64).max())
+    print("C1 max |gpu - fp64 oracle| = %.3e (RGBA32F)" % e64, flush=True)
+    assert m <= 1
+    assert stats["samples"] == s_ref
+    assert same == 1.0
+    vol.close()
+
+
+def test_config_fp16_matches_oracle(ctx, c1_host):
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F16, brick=32)
+    vol.upload(c1_host)
+    h16 = c1_host.astype(np.float16).astype(np.float32)
+    assert np.array_equal(vol.download(), h16)
+    lut = configs.tf_lut(1)
+    st = configs.state(1, 192, 160)
+    img8, img32, stats = _render(ctx, vol, lut, ivr.config_desc(st), 192, 160)
+    ref32, s_ref = oracle.render(h16, oracle.view_of(st), lut, oracle.params())
+    m, rmse, same = _report("fp16 config vs oracle", img8, ref32, img32)
+    assert m <= 1 and stats["samples"] == s_ref
+    vol.close()
+
+
+def test_bricking_and_skipping_are_invisible(ctx):
+    """2x2x2 / 4x4x4 bricks == one brick; macrocell skip on == off, bit for bit"""
+    gp = configs.gen_params(2)
+    n = 128
+    host = ivr.generate_host(gp, n, n, n)
+    lut = configs.tf_lut(2)
+    st = configs.state(2, 200, 176)
+    outs = []
+    for brick, skip in ((128, False), (64, False), (32, True), (128, True)):
+        vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=brick)
+        vol.upload(host)
+        outs.append(_render(ctx, vol, lut, ivr.config_desc(st, skip=skip), 200, 176))
+        vol.close()
+    base8, base32, base_st = outs[0]
+    for img8, img32, s in outs[1:]:
+        assert np.array_equal(img32.view(np.uint32), base32.view(np.uint32))
+        assert s["samples"] == base_st["samples"]
+    print("samples %d, fetched with skipping %d" % (outs[3][2]["samples"], outs[3][2]["visited"]))
+    assert outs[3][2]["visited"] < outs[3][2]["samples"]
+    ref32, s_ref = oracle.render(host, oracle.view_of(st), lut, oracle.params())
+    _report("128^3 noise config vs oracle", base8, ref32, base32)
+    assert s_ref == base_st["samples"]
+
+
+def test_sort_last_matches_oracle_tiles(ctx):
+    """per-brick tiles composited in visibility order == the oracle's tile
+    renders composited by compositetiles, bit for bit"""
+    gp = configs.gen_params(2)
+    n, B = 64, 16
+    host = ivr.generate_host(gp, n, n, n)
+    lut = configs.tf_lut(2)
+    st = configs.state(2, 96, 80)
+    vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=B)
+    vol.upload(host)
+    img8, img32, stats = _render(ctx, vol, lut, ivr.config_desc(st, composite=ivr.SORT_LAST), 96, 80)
+    vol.close()
+    v = oracle.view_of(st)
+    import ctypes as C
+    cam = np.zeros(23, np.float32)
+    rect = (C.c_int * 4)()
+    bmin, bmax = oracle.box(n, n, n)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
+    eb = [int(np.floor((cam[r] - bmin[r]) * (n / (bmax[r] - bmin[r])) / B)) for r in range(3)]
+    nb = (n - 2) // B + 1
+    ids = sorted([(x, y, z) for z in range(nb) for y in range(nb) for x in range(nb)],
+                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb * (b[1] + nb * b[2])))
+    tiles, s_ref = [], 0
+    for b in ids:
+        t, s = oracle.render(host, v, lut, oracle.params(brick=B, brick_id=b))
+        tiles.append(t)
+        s_ref += s
+    ref32 = oracle.composite_tiles(np.stack(tiles))
+    m, rmse, same = _report("sort-last vs oracle tiles", img8, ref32, img32)
+    assert same == 1.0 and stats["samples"] == s_ref
+    # and within the ERT bound of the fused (global ERT) render
+    vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=B)
+    vol.upload(host)
+    f8, f32, _ = _render(ctx, vol, lut, ivr.config_desc(st), 96, 80)
+    vol.close()
+    assert np.abs(f32 - img32).max() <= 0.0101
+
+
+def test_c2_full_size_rows_match_oracle(ctx):
+    """C2 at its full size: 512^3 generated on the GPU, 1024^2 frame; a band
+    of rows checked against the oracle, determinism of repeated renders"""
+    cfg = configs.CONFIGS[2]
+    vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
+    vol.generate(configs.gen_params(2))
+    info = vol.info()
+    assert info["nbricks"] == (2, 2, 2)
+    host = vol.download()
+    assert abs(float(np.mean(host == 0.0)) - 0.4) < 0.005
+    lut = configs.tf_lut(2)
+    st = configs.state(2)
+    d = ivr.config_desc(st)
+    a8, a32, s1 = _render(ctx, vol, lut, d, 1024, 1024)
+    b8, b32, s2 = _render(ctx, vol, lut, d, 1024, 1024)
+    assert np.array_equal(a32.view(np.uint32), b32.view(np.uint32)) and s1["samples"] == s2["samples"]
+    rows = (448, 512)
+    ref32, _ = oracle.render(host, oracle.view_of(st), lut, oracle.params(), rows=rows)
+    ref64, _ = oracle.render(host, oracle.view_of(st), lut, oracle.params(), rows=rows, f64=True)
+    sl = slice(*rows)
+    m, rmse, same = _report("C2 rows %s vs oracle" % (rows,), a8[sl], ref32[sl], a32[sl])
+    print("C2 max |gpu - fp64 oracle| = %.3e" % float(np.abs(a32[sl] - ref64[sl]).max()), flush=True)
+    assert m <= 1
+    assert (a8[..., 3] > 0).mean() > 0.2
+    vol.close()
+
+
+def test_error_paths(ctx, c1_host):
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    vol.upload(c1_host)
+    tf = ctx.tf(configs.tf_lut(1))
+    fr = ctx.frame(64, 64)
+    st = configs.state(1, 128, 128)  # state size does not match the frame
+    with pytest.raises(ivr.IvrError, match="does not match"):
+        ctx.render(vol, tf, ivr.config_desc(st), fr)
+    with pytest.raises(ivr.IvrError):
+        ctx.volume(64, 64, 64, ivr.IVR_F32, brick=48)
+    with pytest.raises(ivr.IvrError):
+        ctx.tf(np.zeros((5000, 4), np.float32))
+    fr.close()
+    tf.close()
+    vol.close()

@@ -1,0 +1,89 @@
+"""CPU-only checks of libivr.so: it loads, exports every entry point the
+public headers declare, and its host-side code (generator, transfer-function
+builder, steering states) behaves as specified.  No GPU call is made."""
+import ctypes as C
+import glob
+import hashlib
+import json
+import math
+import os
+import re
+
+import numpy as np
+
+import ivr_amd as ivr
+from ivr_amd import configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(h).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w \*]*?\b(\w+)\s*\([^;{]*\)\s*;", src, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    names = declared_functions()
+    assert len(names) >= 25, names
+    lib = C.CDLL(ivr.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_host_generator_deterministic_and_pinned():
+    gp = configs.gen_params(1)
+    gp.value_scale = 255.0
+    a = ivr.generate_host(gp, 64, 64, 64)
+    b = ivr.generate_host(gp, 64, 64, 64)
+    assert np.array_equal(a, b)
+    meta = json.load(open(os.path.join(ROOT, "tests", "golden", "golden_meta.json")))
+    assert hashlib.sha256(a.tobytes()).hexdigest() == meta["sha256"]
+    assert a.min() == 0.0 and a.max() == 255.0
+
+
+def test_generator_exp_and_noise_properties():
+    # blobs only, one blob at the centre: the field is a radial Gaussian
+    gp = ivr.gen_params(ivr.GEN_BLOBS, seed=7, nblobs=1)
+    v = ivr.generate_host(gp, 33, 33, 33)
+    c = np.unravel_index(np.argmax(v), v.shape)
+    assert v[c] == 1.0 and v.min() == 0.0
+    # noise with a quantile threshold zeroes that fraction
+    gp = ivr.gen_params(ivr.GEN_NOISE_BLOBS, seed=9, nblobs=4, empty_frac=0.7)
+    v = ivr.generate_host(gp, 40, 40, 40)
+    assert abs(np.mean(v == 0.0) - 0.7) < 0.002 and v.max() == 1.0
+
+
+def test_tf_builder_closed_form():
+    lut = ivr.tf_build(2002, 0.05)
+    assert lut.shape == (256, 4)
+    assert lut[0, 3] == 0.0
+    assert (lut >= 0).all() and (lut[:, :3] <= 1).all() and (lut[:, 3] <= 0.05 + 1e-7).all()
+    # piecewise linear with at most 3 interior knots: second differences vanish
+    # except near the knots
+    d2 = np.abs(np.diff(lut.astype(np.float64), 2, axis=0)).max(axis=1)
+    assert (d2 > 1e-5).sum() <= 6
+    j = ivr.tf_build(2002, 0.05, jitter=0.05, jitter_seed=5)
+    assert np.abs(j - lut).max() <= 0.05 + 0.05 * 255 / 1.0  # positions move, values stay bounded
+    assert j[0, 3] == 0.0
+
+
+def test_orbit_state_is_the_clients_modelview():
+    st = ivr.orbit_state(3002, 1024, 1024)
+    m = np.array(st.mvm, np.float64).reshape(4, 4).T  # column-major -> rows
+    R = m[:3, :3]
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-6)
+    assert np.allclose(m[:3, 3], [0, 0, -5]) and np.allclose(m[3], [0, 0, 0, 1])
+    t = math.tan(math.radians(43.6028) / 2) * 5
+    assert np.allclose(st.vv, [-t, t, -t, t], rtol=1e-6) and abs(t - 2.0) < 1e-4
+    p = np.array(st.pm, np.float64).reshape(4, 4).T
+    assert abs(p[1, 1] - 1 / math.tan(math.radians(43.6028) / 2)) < 1e-5 and p[3, 2] == -1.0
+
+
+def test_state_layout():
+    assert C.sizeof(ivr.State) == 184
+    assert ivr.State.mvm.offset == 56 and ivr.State.pm.offset == 120
