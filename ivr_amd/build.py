@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libivr.so")
+COMPAT_LIB = os.path.join(HERE, "libivr_compat.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IVR_ARCH", "gfx950")
 
@@ -35,9 +36,15 @@ def _run(cmd):
 
 
 def sources():
+    """(hip, c) sources of libivr.so; compat* files form libivr_compat.so"""
     hip = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
     c = sorted(f for f in os.listdir(CSRC) if f.endswith(".c"))
     return hip, c
+
+
+def _split(files):
+    return ([f for f in files if not f.startswith("compat")],
+            [f for f in files if f.startswith("compat")])
 
 
 def build(verbose=False):
@@ -54,10 +61,18 @@ def build(verbose=False):
         for r in ex.map(_run, jobs):
             if verbose and (r.stdout or r.stderr):
                 sys.stderr.write(r.stdout + r.stderr)
-    objs = [os.path.join(OUT, f + ".o") for f in hip + c]
+    core_h, compat_h = _split(hip)
+    core_c, compat_c = _split(c)
+    objs = [os.path.join(OUT, f + ".o") for f in core_h + core_c]
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-lm"])
     os.replace(tmp, LIB)
+    # drop-in library: reference names, host globals resolved at load time
+    objs = [os.path.join(OUT, f + ".o") for f in compat_h + compat_c]
+    tmp = COMPAT_LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-L" + HERE, "-livr", "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(tmp, COMPAT_LIB)
     return LIB
 
 

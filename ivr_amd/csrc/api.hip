@@ -226,14 +226,12 @@ int ivr_frame_stats_get(ivr_ctx *ctx, ivr_frame *f, ivr_frame_stats *st)
 
 /* ---------------- render ---------------- */
 
-namespace {
-
-void fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
-               const ivr_frame_desc *d, ivr_frame *f)
+void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
+                   const ivr_frame_desc *d, ivr_frame *f)
 {
     memset(&a, 0, sizeof a);
-    a.W = f->W;
-    a.H = f->H;
+    a.W = f ? f->W : 0;
+    a.H = f ? f->H : 0;
     for (int r = 0; r < 4; r++) a.rect[r] = cam.rect[r];
     a.mode = d->mode;
     a.xfer = d->xfer;
@@ -274,10 +272,12 @@ void fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr
     a.alpha = d->state.alpha;
     a.step_vox = d->step_vox;
     a.ert = d->ert;
-    a.out8 = f->d_rgba8;
-    a.out32 = f->d_rgba32;
-    a.counters = f->d_counters;
+    a.out8 = f ? f->d_rgba8 : nullptr;
+    a.out32 = f ? f->d_rgba32 : nullptr;
+    a.counters = f ? f->d_counters : nullptr;
 }
+
+namespace {
 
 /* pixel rectangle of a brick's ownership box grown by one voxel, projected
  * with pm * mvm like updatevolbbox (mathutil.c:948-1018); whole image when
@@ -400,7 +400,7 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
     ivr_cam cam;
     ivr_cam_setup(&d->state, v->n[0], v->n[1], v->n[2], &cam);
     RenderArgs a;
-    fill_args(a, cam, v, tf, d, f);
+    ivr_fill_args(a, cam, v, tf, d, f);
     IVR_HIP(hipMemsetAsync(f->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
     f->timed = 1;
     f->composited = 0;

@@ -112,6 +112,10 @@ struct RenderArgs {
     float4 *tile;
 };
 
+/* kernel arguments of one frame (api.hip); f may be null for per-ray calls */
+void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
+                   const ivr_frame_desc *d, ivr_frame *f);
+
 /* launchers (render.hip) */
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);
 int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a);

@@ -1,0 +1,371 @@
+/*
+ * render_dev.h -- device helpers of the render kernels, shared by render.hip
+ * (frame kernels) and compat.hip (reference entry points run per ray or per
+ * sample).  Every helper restates one piece of the reference render path in
+ * its float operation order; contraction is off for the whole library, so
+ * the only fused multiply-adds are the explicit fmaf calls.
+ */
+#ifndef IVR_RENDER_DEV_H
+#define IVR_RENDER_DEV_H
+
+#include <float.h>
+
+#include "ivr_internal.h"
+
+namespace ivr_dev {
+
+typedef _Float16 h16;
+
+template <typename VT>
+__device__ __forceinline__ float ldv(const VT *p) { return (float)(*p); }
+
+/* 8-voxel footprint at global floor index (j,i,k), j along x.  Order:
+ * (j,i,k) (j+1,i,k) (j,i+1,k) (j+1,i+1,k) then the same at k+1. */
+template <typename VT>
+__device__ __forceinline__ void fetch8(const RenderArgs &a, const unsigned long long *s_boff, int j,
+                                       int i, int k, float v[8])
+{
+    const int bid = (j >> a.bshift) + a.nb[0] * ((i >> a.bshift) + a.nb[1] * (k >> a.bshift));
+    const int P = a.P, PP = a.P * a.P;
+    const VT *p = (const VT *)a.vox_data + s_boff[bid] +
+                  ((((k & a.bmask) + 1) * P + ((i & a.bmask) + 1)) * P + ((j & a.bmask) + 1));
+    v[0] = ldv(p);
+    v[1] = ldv(p + 1);
+    v[2] = ldv(p + P);
+    v[3] = ldv(p + P + 1);
+    v[4] = ldv(p + PP);
+    v[5] = ldv(p + PP + 1);
+    v[6] = ldv(p + PP + P);
+    v[7] = ldv(p + PP + P + 1);
+}
+
+__device__ __forceinline__ float dot3n(float n0, float n1, float n2, const float *v)
+{
+    return n0 * v[0] + n1 * v[1] + n2 * v[2];
+}
+
+/* getrayintersect (ray.c:276-318): planes -x,+x,-y,+y,-z,+z */
+__device__ __forceinline__ bool slab(const float *dc, const float *o, const float *d, float &tin,
+                                     float &tout)
+{
+    float ti = -FLT_MAX, to = FLT_MAX;
+#pragma unroll
+    for (int f = 0; f < 6; f++) {
+        if (!(ti < to)) break;
+        const float s = (f & 1) ? 1.0f : -1.0f;
+        const float n0 = (f >> 1) == 0 ? s : 0.0f;
+        const float n1 = (f >> 1) == 1 ? s : 0.0f;
+        const float n2 = (f >> 1) == 2 ? s : 0.0f;
+        float numer = dc[f] - dot3n(n0, n1, n2, o);
+        float denom = dot3n(n0, n1, n2, d);
+        if (denom == 0.0f) {
+            if (numer < 0.0f) ti = to + 1.0f;
+        } else {
+            float th = numer / denom;
+            if (denom > 0.0f) to = fminf(to, th);
+            else ti = fmaxf(ti, th);
+        }
+    }
+    tin = ti;
+    tout = to;
+    return (ti < to) && (ti > 0.0f || to > 0.0f);
+}
+
+__device__ __forceinline__ float lerp3(float a, float b, float t) { return fmaf(t, b - a, a); }
+
+__device__ __forceinline__ float trilinear(const float v[8], float fx, float fy, float fz)
+{
+    float c00 = lerp3(v[0], v[1], fx);
+    float c10 = lerp3(v[2], v[3], fx);
+    float c01 = lerp3(v[4], v[5], fx);
+    float c11 = lerp3(v[6], v[7], fx);
+    float c0 = lerp3(c00, c10, fy);
+    float c1 = lerp3(c01, c11, fy);
+    return lerp3(c0, c1, fz);
+}
+
+__device__ __forceinline__ int lut_index(const RenderArgs &a, float s)
+{
+    int idx = (int)(s * a.lutscale) - a.lutoffset;
+    return min(max(idx, 0), a.lutsize - 1);
+}
+
+/* config-mode sample position: global grid t_k = tin + k dt, voxel coords
+ * clamped to [0, n-1], floor index clamped to n-2 */
+struct Pos {
+    int j, i, k;
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ Pos cfg_pos(const RenderArgs &a, const float *o, const float *d, float tin,
+                                       float dt, int kk)
+{
+    const float t = fmaf((float)kk, dt, tin);
+    float gx = (fmaf(d[0], t, o[0]) - a.bmin[0]) * a.s[0];
+    float gy = (fmaf(d[1], t, o[1]) - a.bmin[1]) * a.s[1];
+    float gz = (fmaf(d[2], t, o[2]) - a.bmin[2]) * a.s[2];
+    gx = fminf(fmaxf(gx, 0.0f), a.nm1f[0]);
+    gy = fminf(fmaxf(gy, 0.0f), a.nm1f[1]);
+    gz = fminf(fmaxf(gz, 0.0f), a.nm1f[2]);
+    Pos p;
+    p.j = min((int)gx, a.n[0] - 2);
+    p.i = min((int)gy, a.n[1] - 2);
+    p.k = min((int)gz, a.n[2] - 2);
+    p.fx = gx - (float)p.j;
+    p.fy = gy - (float)p.i;
+    p.fz = gz - (float)p.k;
+    return p;
+}
+
+__device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
+{
+    return (p.j >> IVR_CELL_SHIFT) +
+           a.nc[0] * ((p.i >> IVR_CELL_SHIFT) + a.nc[1] * (p.k >> IVR_CELL_SHIFT));
+}
+
+/* per-pixel config-mode ray direction: M (u, v, 5) */
+__device__ __forceinline__ void cfg_dir(const RenderArgs &a, int i, int j, float *d)
+{
+    const float uu = fmaf((float)j + 0.5f, a.du, a.l);
+    const float vv = fmaf((float)i + 0.5f, a.dv, a.b);
+#pragma unroll
+    for (int r = 0; r < 3; r++) d[r] = fmaf(a.M[r * 3 + 0], uu, fmaf(a.M[r * 3 + 1], vv, a.m5[r]));
+}
+
+struct Acc {
+    float r, g, b, a;
+};
+
+/* front-to-back march of one ray over the global sample grid */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *s_lut,
+                                             const unsigned long long *s_boff, const float *o,
+                                             const float *d, Acc &acc, unsigned &cnt,
+                                             unsigned &fetched)
+{
+    float tin, tout;
+    if (!slab(a.dconst, o, d, tin, tout)) return;
+    tin = fmaxf(tin, 0.0f);
+    const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    const float dt = (a.vox * a.step_vox) / dlen;
+    const float nf = 1.0f + (tout - tin) / dt + 0.5f;
+    /* guard for degenerate directions; never reached by a finite ray */
+    const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
+    /* voxel-space ray for macrocell exits: g(t) ~ go + gd t */
+    float go[3], rgd[3], gd[3];
+    if (SKIP) {
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            go[r] = (o[r] - a.bmin[r]) * a.s[r];
+            gd[r] = d[r] * a.s[r];
+            rgd[r] = 1.0f / gd[r];
+        }
+    }
+    int k = 0;
+    while (k < n) {
+        const Pos p = cfg_pos(a, o, d, tin, dt, k);
+        if (SKIP) {
+            const int cell = cell_of(a, p);
+            if (a.cellempty[cell]) {
+                /* every sample of this cell is transparent: jump to the first
+                 * sample past the cell exit, verified exactly on the grid */
+                const int cj[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
+                float te = FLT_MAX;
+#pragma unroll
+                for (int r = 0; r < 3; r++) {
+                    if (gd[r] != 0.0f) {
+                        const float face = (float)((cj[r] + (gd[r] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
+                        te = fminf(te, (face - go[r]) * rgd[r]);
+                    }
+                }
+                int kc = k + 1;
+                const float q = (te - tin) / dt;
+                if (q > (float)(k + 1)) {
+                    int cand = q < (float)n ? (int)q : n;
+                    if (cand > k + 1) {
+                        const Pos pv = cfg_pos(a, o, d, tin, dt, cand - 1);
+                        if (cell_of(a, pv) == cell) kc = cand;
+                    }
+                }
+                cnt += (unsigned)(kc - k);
+                k = kc;
+                continue;
+            }
+        }
+        float v[8];
+        fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+        const float s = trilinear(v, p.fx, p.fy, p.fz);
+        const float4 e = s_lut[lut_index(a, s)];
+        const float al = e.w * a.alpha;
+        const float w = (1.0f - acc.a) * al;
+        acc.r = fmaf(w, e.x, acc.r);
+        acc.g = fmaf(w, e.y, acc.g);
+        acc.b = fmaf(w, e.z, acc.b);
+        acc.a = acc.a + w;
+        cnt++;
+        fetched++;
+        k++;
+        if (acc.a >= a.ert) break;
+    }
+}
+
+/* getlinearsample, non-BGQ (ray.c:122-178): reference weights and axes */
+template <typename VT>
+__device__ __forceinline__ float sample_ref(const RenderArgs &a, const unsigned long long *s_boff,
+                                            float x, float y, float z)
+{
+    float ratio = (x - a.bmin[0]) / a.ext[0];
+    int j = (int)((float)a.n[0] * ratio);
+    ratio = (y - a.bmin[1]) / a.ext[1];
+    int i = (int)((float)a.n[1] * ratio);
+    ratio = (z - a.bmin[2]) / a.ext[2];
+    int k = (int)((float)a.n[2] * ratio);
+    i = max(min(i, a.n[1] - 2), 0);
+    j = max(min(j, a.n[0] - 2), 0);
+    k = max(min(k, a.n[2] - 2), 0);
+    float v[8];
+    fetch8<VT>(a, s_boff, j, i, k, v);
+    const float xp = x + fabsf(a.bmin[0]);
+    const float yp = y + fabsf(a.bmin[1]);
+    const float zp = z + fabsf(a.bmin[2]);
+    const float x0 = ((float)j * a.ext[0]) / (float)a.n[0];
+    const float y0 = ((float)i * a.ext[1]) / (float)a.n[1];
+    const float z0 = ((float)k * a.ext[2]) / (float)a.n[2];
+    const float xd = (fabsf(xp - x0) * a.ext[0]) / (float)a.n[0];
+    const float yd = (fabsf(yp - y0) * a.ext[1]) / (float)a.n[1];
+    const float zd = (fabsf(zp - z0) * a.ext[2]) / (float)a.n[2];
+    /* v000=v[0], v100=(j,i+1)=v[2], v010=(j+1,i)=v[1], v110=v[3], ... */
+    const float c00 = v[0] * (1.0f - xd) + v[2] * xd;
+    const float c10 = v[1] * (1.0f - xd) + v[3] * xd;
+    const float c01 = v[4] * (1.0f - xd) + v[6] * xd;
+    const float c11 = v[5] * (1.0f - xd) + v[7] * xd;
+    const float c0 = c00 * (1.0f - yd) + c10 * yd;
+    const float c1 = c01 * (1.0f - yd) + c11 * yd;
+    return c0 * (1.0f - zd) + c1 * zd;
+}
+
+/* one step of transfer function `x` (transferfuncs.c:24-199) with sample
+ * value s onto colour c; lut is the RGBA table (LDS or global) */
+__device__ __forceinline__ void tf_step(const RenderArgs &a, const float4 *s_lut, int x, float s,
+                                        Acc &c)
+{
+    if (x == 2 || x == 3) { /* SEISMIC2 / CHEM (transferfuncs.c:109-168) */
+        int idx = (int)(s * a.lutscale) - a.lutoffset;
+        if (idx < 0) idx = 0;
+        if (x == 2 && idx > a.maxlutindx) idx = a.maxlutindx;
+        idx = min(idx, a.lutsize - 1);
+        const float4 e = s_lut[idx];
+        const float al = e.w * a.alpha;
+        const float om = 1.0f - al;
+        c.b = om * c.b + al * e.z;
+        c.g = om * c.g + al * e.y;
+        c.r = om * c.r + al * e.x;
+        c.a = al + om * c.a;
+        if (c.a > 1.0f) c.a = 1.0f;
+    } else if (x == 5) { /* RGBBLEND (transferfuncs.c:170-199) */
+        const unsigned u = __float_as_uint(s);
+        const float r = 1.0f / 255.0f * (float)((u >> 24) & 0xff);
+        const float g = 1.0f / 255.0f * (float)((u >> 16) & 0xff);
+        const float b = 1.0f / 255.0f * (float)((u >> 8) & 0xff);
+        const float al = a.alpha, om = 1.0f - al;
+        c.b = om * c.b + al * b;
+        c.g = om * c.g + al * g;
+        c.r = om * c.r + al * r;
+        c.a = al + om * c.a;
+        if (c.a > 1.0f) c.a = 1.0f;
+    } else if (x == 1) { /* MEDXFER (transferfuncs.c:24-68) */
+        const int idx = min(max((int)(s * a.lutscale) - a.lutoffset, 0), a.lutsize - 1);
+        const float4 e = s_lut[idx];
+        c.b = ((double)e.z < 0.8) ? fmaxf(e.z, c.b) : c.b;
+        c.g = ((double)e.y < 0.8) ? fmaxf(e.y, c.g) : c.g;
+        c.r = ((double)e.x < 0.8) ? fmaxf(e.x, c.r) : c.r;
+        c.a = e.w * a.alpha;
+    } else if (x == 4) { /* BLEND (transferfuncs.c:70-84): no lutoffset */
+        const int idx = min(max((int)(s * a.lutscale), 0), a.lutsize - 1);
+        const float4 e = s_lut[idx];
+        const float al = e.w;
+        c.b = al * e.z + (1.0f - al) * c.b;
+        c.g = al * e.y + (1.0f - al) * c.g;
+        c.r = al * e.x + (1.0f - al) * c.r;
+        c.a = al * a.alpha;
+    } else { /* SEISMIC (transferfuncs.c:86-107) */
+        const int idx = min(max((int)(s * a.lutscale) - a.lutoffset, 0), a.lutsize - 1);
+        const float4 e = s_lut[idx];
+        const double lb = e.z, lg = e.y, lr = e.x;
+        c.b = ((lb > 0.65) && (e.z < 1.0f)) ? 0.6f : c.b;
+        c.g = ((lg > 0.65) && (e.y < 1.0f)) ? 0.0f : c.g;
+        c.r = ((lr > 0.65) && (e.x < 1.0f)) ? 0.0f : c.r;
+        c.b = ((lb > 0.5) && (e.z < 0.65f)) ? 0.9f : c.b;
+        c.g = ((lg > 0.5) && (e.y < 0.65f)) ? 0.9f : c.g;
+        c.r = ((lr > 0.5) && (e.x < 0.65f)) ? 0.9f : c.r;
+        c.b = ((lb > 0.3) && (e.z < 0.5f)) ? 0.0f : c.b;
+        c.g = ((lg > 0.3) && (e.y < 0.5f)) ? 0.0f : c.g;
+        c.r = ((lr > 0.3) && (e.x < 0.5f)) ? 0.6f : c.r;
+        c.a = 1.0f * a.alpha;
+    }
+}
+
+/* transfer functions that walk the samples from the far end (BTF) */
+__device__ __forceinline__ bool tf_backward(int x) { return x == 2 || x == 3 || x == 5; }
+
+/* one compat-mode ray: getsamplesonray + the selected transfer function */
+template <typename VT>
+__device__ void march_compat(const RenderArgs &a, const float4 *s_lut, const unsigned long long *s_boff,
+                             const float *o, const float *d, Acc &c, unsigned &cnt)
+{
+    float tin, tout;
+    if (!slab(a.dconst, o, d, tin, tout)) return;
+    const float dt = 0.001f; /* DELTASAMPLE (types.h:17) */
+    int iend = 1.0f + (tout - tin) / dt + 0.5f;
+    iend = iend < 256 ? iend : 256;
+    if (iend <= 0) return;
+    cnt += (unsigned)iend;
+    const int x = a.xfer;
+    const bool backward = tf_backward(x);
+    for (int q = 0; q < iend; q++) {
+        const int s_i = backward ? iend - 1 - q : q;
+        const float t = tin + ((float)s_i * dt);
+        const float px = d[0] * t + o[0];
+        const float py = d[1] * t + o[1];
+        const float pz = d[2] * t + o[2];
+        tf_step(a, s_lut, x, sample_ref<VT>(a, s_boff, px, py, pz), c);
+    }
+}
+
+__device__ __forceinline__ unsigned char quant8(float f)
+{
+    f = fminf(fmaxf(f, 0.0f), 1.0f);
+    return (unsigned char)((double)f * 255.0 + 0.5);
+}
+
+__device__ __forceinline__ void stage(const RenderArgs &a, float4 *s_lut, unsigned long long *s_boff)
+{
+    for (int t = threadIdx.x; t < a.lutsize; t += blockDim.x) s_lut[t] = a.lut[t];
+    for (int t = threadIdx.x; t < a.nbricks; t += blockDim.x) s_boff[t] = a.boff[t];
+    __syncthreads();
+}
+
+/* XCD-aware workgroup -> 16x16 pixel block; returns false for padding */
+__device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
+{
+    const int b = blockIdx.x;
+    const int per = a.nblk_pad >> 3;
+    const int L = (b & 7) * per + (b >> 3);
+    if (L >= a.nblk) return false;
+    const int bx = L % a.nblk_x, by = L / a.nblk_x;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    j = bx * 16 + (w & 1) * 8 + (lane & 7);
+    i = by * 16 + (w >> 1) * 8 + (lane >> 3);
+    return true;
+}
+
+__device__ __forceinline__ void count_add(unsigned long long *ctr, unsigned v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+}
+
+} // namespace ivr_dev
+
+#endif

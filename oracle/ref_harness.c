@@ -45,7 +45,19 @@ boolean_t getrayintersect(ray3f, float *, float *);
 float getlinearsample(float, float, float);
 void initsamplefunc(int);
 
+/* present when linked against the drop-in libivr_compat.so */
+extern int ivr_compat_createlocalimage(int) __attribute__((weak));
+
 static color4f *g_user_lut = NULL;
+static int g_frame_gpu = 0;
+
+/* 1: render frames through ivr_compat_createlocalimage (one GPU launch)
+ * instead of the reference createlocalimage loop; returns whether available */
+int ivrref_frame_on_gpu(int on)
+{
+    g_frame_gpu = on && ivr_compat_createlocalimage != NULL;
+    return g_frame_gpu;
+}
 
 /* init(): volume file of nx*ny*nz little-endian float32, x fastest. */
 int ivrref_init(const char *volfile, int nx, int ny, int nz, int cmap, int xfer,
@@ -117,7 +129,12 @@ int ivrref_render(const float mvm_cm[16], const float pm_cm[16], const float vv[
      * touches the top edge (main.c:960); give it room. */
     free(colorbuf);
     colorbuf = (color4f *)calloc((size_t)W * (size_t)(H + 2), sizeof(color4f));
-    createlocalimage(0);
+    if (g_frame_gpu) {
+        memset(colorbuf, 0, sizeof(color4f) * (size_t)W * (size_t)H);
+        if (ivr_compat_createlocalimage(0) != 0) return -1;
+    } else {
+        createlocalimage(0);
+    }
     memcpy(out, colorbuf, sizeof(color4f) * (size_t)W * (size_t)H);
     rect[0] = (int)(aabb.wincoord.y - 0.5f);
     rect[1] = (int)(aabb.wincoord.y + aabb.wincoord.h + 1.0f);

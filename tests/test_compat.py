@@ -1,0 +1,132 @@
+"""The drop-in library libivr_compat.so (include/ivr_compat.h).
+
+oracle/_ref/libivrref_gpu.so is the reference's own main.c, mathutil.c and
+utils.c (compiled from its sources) linked against libivr_compat.so in place
+of ray.c, transferfuncs.c and colormap.c.  Driven by the same harness as the
+golden vectors, it must reproduce them bit for bit: whole frames (through the
+reference's createlocalimage calling our raycast per pixel, and through
+ivr_compat_createlocalimage as one launch), samplers, slab tests, sample
+marching, transfer functions and colormaps.
+"""
+import ctypes as C
+import hashlib
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ivr_amd as ivr
+from ivr_amd import configs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+G = os.path.join(ROOT, "tests", "golden")
+META = json.load(open(os.path.join(G, "golden_meta.json")))
+COMPAT = os.path.join(ROOT, "ivr_amd", "libivr_compat.so")
+REFGPU = os.path.join(ROOT, "oracle", "_ref", "libivrref_gpu.so")
+
+
+def f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def test_compat_exports_reference_names():
+    src = open(os.path.join(ROOT, "include", "ivr_compat.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set(re.findall(r"^[\w ]+?\b(\w+)\s*\([^;{]*\)\s*;", src, flags=re.M))
+    out = subprocess.run(["nm", "-D", "--defined-only", COMPAT], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    assert {"raycast", "getsamplesonray", "chemistrytransferfunc", "medcolormap"} <= names
+    missing = sorted(names - exported)
+    assert not missing, missing
+    for g in ("dconst", "getsamplefunc", "transferfunc", "colormap"):
+        assert g in exported, g
+
+
+def _load_refgpu():
+    if not os.path.exists(REFGPU):
+        pytest.skip("oracle/_ref/libivrref_gpu.so not built (needs the reference sources)")
+    lib = C.CDLL(REFGPU)
+    lib.ivrref_getlinearsample.restype = C.c_float
+    return lib
+
+
+def test_colormaps_through_dropin():
+    lib = _load_refgpu()
+    z = np.load(os.path.join(G, "colormaps.npz"))
+    for key in z.files:
+        cid, hsize = (int(t) for t in key[3:].split("_"))
+        hmin, hmax = (0.0, 255.0) if hsize == 256 else (1468.0, 4985.0)
+        out = np.zeros((hsize, 4), np.float32)
+        lib.ivrref_colormap(cid, hsize, C.c_float(hmin), C.c_float(hmax), f(out))
+        assert np.array_equal(out.view(np.uint32), z[key].view(np.uint32)), key
+
+
+@pytest.fixture(scope="module")
+def refgpu():
+    lib = _load_refgpu()
+    gp = configs.gen_params(1)
+    gp.value_scale = 255.0
+    vol = ivr.generate_host(gp, 64, 64, 64)
+    assert hashlib.sha256(vol.tobytes()).hexdigest() == META["sha256"]
+    path = "/tmp/ivr_compat_vol.raw"
+    vol.tofile(path)
+    assert lib.ivrref_init(path.encode(), 64, 64, 64, -1, 3, 8) == 0
+    return lib
+
+
+@pytest.mark.gpu
+def test_frames_through_dropin(refgpu):
+    z = np.load(os.path.join(G, "compat_images.npz"))
+    luts = {"tf5": z["lut_tf5"], "medical": z["lut_medical"], "jet": z["lut_jet"]}
+    W, H = META["width"], META["height"]
+    for use_gpu_frame in (1, 0):
+        assert refgpu.ivrref_frame_on_gpu(use_gpu_frame) == use_gpu_frame
+        cases = list(enumerate(META["compat_cases"]))
+        if not use_gpu_frame:
+            cases = cases[:2]  # the per-pixel path launches once per ray
+        for k, case in cases:
+            print("drop-in frame", use_gpu_frame, k, case, flush=True)
+            st = ivr.state_from_angles(case["az"], case["el"], W, H)
+            lut = luts[case["lut"]]
+            refgpu.ivrref_set_lut(f(lut), 256, 0, case["maxlutindx"])
+            refgpu.ivrref_set_xfer(case["xfer"])
+            out = np.zeros((H, W, 4), np.float32)
+            rect = (C.c_int * 4)()
+            rc = refgpu.ivrref_render(f(np.array(st.mvm, np.float32)), f(np.array(st.pm, np.float32)),
+                                      f(np.array(st.vv, np.float32)), f(np.array(st.newconst, np.float32)),
+                                      C.c_float(case["alpha"]), W, H, f(out), rect)
+            assert rc == 0
+            assert np.array_equal(out.view(np.uint32), z["images"][k].view(np.uint32)), (use_gpu_frame, case)
+    refgpu.ivrref_frame_on_gpu(0)
+
+
+@pytest.mark.gpu
+def test_per_ray_functions_through_dropin(refgpu):
+    z = np.load(os.path.join(G, "sampler.npz"))
+    vals = np.array([refgpu.ivrref_getlinearsample(C.c_float(p[0]), C.c_float(p[1]), C.c_float(p[2]))
+                     for p in z["points"][:60]], np.float32)
+    assert np.array_equal(vals.view(np.uint32), z["values"][:60].view(np.uint32))
+    z = np.load(os.path.join(G, "slab.npz"))
+    for r, hit, tin, tout in list(zip(z["rays"], z["hit"], z["tin"], z["tout"]))[:60]:
+        ti, to = C.c_float(), C.c_float()
+        assert refgpu.ivrref_getrayintersect(f(np.ascontiguousarray(r)), C.byref(ti), C.byref(to)) == hit
+        assert np.float32(ti.value).tobytes() == np.float32(tin).tobytes()
+        assert np.float32(to.value).tobytes() == np.float32(tout).tobytes()
+    z = np.load(os.path.join(G, "samplesonray.npz"))
+    for r, n, s in list(zip(z["rays"], z["n"], z["samples"]))[:20]:
+        buf = np.zeros(256, np.float32)
+        got = refgpu.ivrref_getsamplesonray(f(np.ascontiguousarray(r)), 256, f(buf))
+        assert got == n
+        if n > 0:
+            assert np.array_equal(buf[:n].view(np.uint32), s[:n].view(np.uint32))
+    z = np.load(os.path.join(G, "transferfuncs.npz"))
+    refgpu.ivrref_set_lut(f(np.ascontiguousarray(z["lut"])), 256, int(z["lutoffset"]), int(z["maxlutindx"]))
+    for x in range(6):
+        for k in range(0, 12, 3):
+            o = np.zeros(4, np.float32)
+            s = np.ascontiguousarray(z["samples"][k])
+            refgpu.ivrref_transferfunc(x, f(s), 64 - k, C.c_float(float(z["alpha"])), f(o))
+            assert np.array_equal(o.view(np.uint32), z["out"][x, k].view(np.uint32)), (x, k)
