@@ -138,6 +138,56 @@ int ivr_frame_stats_get(ivr_ctx *ctx, ivr_frame *fr, ivr_frame_stats *st);
 int ivr_render(ivr_ctx *ctx, const ivr_volume *vol, const ivr_tf *tf,
                const ivr_frame_desc *desc, ivr_frame *fr);
 
+/* ---- learned surrogate: per-brick multiresolution hash grids (16 levels
+ * x 2 features) and one MLP 32 -> 64 -> 64 -> 1 on bf16 MFMA, trained on
+ * samples of a plain volume and rendered in place of its trilinear fetch.
+ * No counterpart in the reference. ---- */
+typedef struct ivr_surrogate ivr_surrogate;
+typedef struct ivr_surrogate_params {
+    int log2_table;   /* hash-table entries per level and brick = 2^log2_table */
+    int base_res;     /* level-0 resolution per brick edge                      */
+    float growth;     /* resolution ratio between levels                        */
+    float lr;         /* Adam step size of the hash tables                      */
+    float lr_mlp;     /* Adam step size of the MLP                              */
+    float beta1, beta2, eps;
+    float init_scale; /* tables start U[-init_scale, init_scale]                */
+    uint64_t seed;
+} ivr_surrogate_params;
+
+void ivr_surrogate_defaults(ivr_surrogate_params *sp);
+int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogate_params *sp,
+                         ivr_surrogate **out);
+int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s);
+/* `steps` Adam steps of `samples` uniform samples (spread over the bricks)
+ * whose targets are the trilinear values of `vol`; loss_out = last MSE */
+int ivr_surrogate_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, int steps,
+                        long long samples, float *loss_out);
+/* gradients of one batch of caller samples (local xyz in [0,1]^3, brick id,
+ * target), without an update: outputs y, MSE, MLP gradient (6337 floats in
+ * the W1 b1 W2 b2 W3 b3 order) and one brick's table gradient */
+int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, const int *bricks,
+                       const float *targets, float *y_out, float *loss_out, float *gmlp_out,
+                       float *gtable_out, int gtable_brick);
+/* one Adam update from the gradients of the last ivr_surrogate_grad */
+int ivr_surrogate_adam(ivr_ctx *ctx, ivr_surrogate *s);
+int ivr_surrogate_get(ivr_ctx *ctx, const ivr_surrogate *s, float *mlp, float *table, int brick);
+int ivr_surrogate_set(ivr_ctx *ctx, ivr_surrogate *s, const float *mlp, const float *table, int brick);
+int ivr_surrogate_info(const ivr_surrogate *s, int *nbricks, unsigned long long *table_entries,
+                       int *step, int res[16]);
+/* render a config-mode frame from the surrogate; bricks flagged for
+ * fallback fetch `vol` trilinearly instead (vol may be NULL if none is) */
+int ivr_surrogate_render(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol,
+                         const ivr_tf *tf, const ivr_frame_desc *desc, ivr_frame *fr);
+/* per-brick image PSNR of the surrogate against the plain render of each
+ * brick alone (RGBA8, dB); bricks below threshold_db fall back */
+int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+                           const ivr_frame_desc *desc, float threshold_db, float *psnr_out,
+                           int *nfallback);
+int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned char *flags);
+/* versioned binary checkpoint of parameters and Adam state */
+int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
+int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
+
 /* ---- host-side helpers (no GPU) ---- */
 /* Defaults for a config-mode frame of the given state. */
 void ivr_desc_config(ivr_frame_desc *d);

@@ -53,6 +53,14 @@ class FrameStats(C.Structure):
                 ("render_ms", C.c_float), ("composite_ms", C.c_float)]
 
 
+class SurrogateParams(C.Structure):
+    _fields_ = [("log2_table", C.c_int), ("base_res", C.c_int), ("growth", C.c_float), ("lr", C.c_float),
+                ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("init_scale", C.c_float), ("seed", C.c_uint64)]
+
+
+SG_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1
+
 assert C.sizeof(State) == 184
 assert C.sizeof(GenParams) == 40
 assert C.sizeof(FrameDesc) == 220
@@ -93,6 +101,20 @@ def _load():
         "ivr_tf_build": (None, [C.c_uint64, C.c_float, C.c_float, C.c_uint64, P]),
         "ivr_generate_host": (C.c_int, [C.POINTER(GenParams), C.c_int, C.c_int, C.c_int, P]),
         "ivr_rng_u32": (C.c_uint32, [C.POINTER(C.c_uint64)]),
+        "ivr_surrogate_defaults": (None, [C.POINTER(SurrogateParams)]),
+        "ivr_surrogate_create": (C.c_int, [P, P, C.POINTER(SurrogateParams), C.POINTER(P)]),
+        "ivr_surrogate_destroy": (C.c_int, [P, P]),
+        "ivr_surrogate_train": (C.c_int, [P, P, P, C.c_int, C.c_longlong, C.POINTER(C.c_float)]),
+        "ivr_surrogate_grad": (C.c_int, [P, P, C.c_int, P, P, P, P, C.POINTER(C.c_float), P, P, C.c_int]),
+        "ivr_surrogate_adam": (C.c_int, [P, P]),
+        "ivr_surrogate_get": (C.c_int, [P, P, P, P, C.c_int]),
+        "ivr_surrogate_set": (C.c_int, [P, P, P, P, C.c_int]),
+        "ivr_surrogate_info": (C.c_int, [P, P, P, P, P]),
+        "ivr_surrogate_render": (C.c_int, [P, P, P, P, C.POINTER(FrameDesc), P]),
+        "ivr_surrogate_evaluate": (C.c_int, [P, P, P, P, C.POINTER(FrameDesc), C.c_float, P, P]),
+        "ivr_surrogate_set_fallback": (C.c_int, [P, P, P]),
+        "ivr_surrogate_save": (C.c_int, [P, P, C.c_char_p]),
+        "ivr_surrogate_load": (C.c_int, [P, P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -217,6 +239,9 @@ class Context:
     def render(self, vol, tf, desc, frame):
         check(lib.ivr_render(self.h, vol.h, tf.h, C.byref(desc), frame.h))
 
+    def surrogate(self, vol, **kw):
+        return Surrogate(self, vol, **kw)
+
 
 class Volume:
     def __init__(self, ctx, nx, ny, nz, dtype, brick):
@@ -300,3 +325,113 @@ class Frame:
         if self.h:
             lib.ivr_frame_destroy(self.ctx.h, self.h)
             self.h = None
+
+
+def surrogate_defaults(**kw):
+    sp = SurrogateParams()
+    lib.ivr_surrogate_defaults(C.byref(sp))
+    for k, v in kw.items():
+        setattr(sp, k, v)
+    return sp
+
+
+class Surrogate:
+    """per-brick hash grids + shared MLP trained on a plain volume"""
+
+    def __init__(self, ctx, vol, **kw):
+        self.ctx = ctx
+        self.sp = surrogate_defaults(**kw)
+        h = C.c_void_p()
+        check(lib.ivr_surrogate_create(ctx.h, vol.h, C.byref(self.sp), C.byref(h)))
+        self.h = h
+        nb, T, step = C.c_int(), C.c_ulonglong(), C.c_int()
+        res = (C.c_int * 16)()
+        check(lib.ivr_surrogate_info(h, C.byref(nb), C.byref(T), C.byref(step), res))
+        self.nbricks, self.T, self.res = nb.value, T.value, list(res)
+
+    def train(self, vol, steps, samples):
+        loss = C.c_float()
+        check(lib.ivr_surrogate_train(self.ctx.h, self.h, vol.h, steps, samples, C.byref(loss)))
+        return loss.value
+
+    def grad(self, xyz, bricks, targets, table_brick=0):
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        bricks = np.ascontiguousarray(bricks, np.int32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        n = len(targets)
+        y = np.zeros(n, np.float32)
+        g = np.zeros(SG_NPARAM, np.float32)
+        gt = np.zeros((16, self.T, 2), np.float32)
+        loss = C.c_float()
+        check(lib.ivr_surrogate_grad(self.ctx.h, self.h, n, _ptr(xyz), _ptr(bricks), _ptr(targets), _ptr(y),
+                                     C.byref(loss), _ptr(g), _ptr(gt), table_brick))
+        return y, loss.value, g, gt
+
+    def adam(self):
+        check(lib.ivr_surrogate_adam(self.ctx.h, self.h))
+
+    def get(self, brick=0, table=True):
+        mlp = np.zeros(SG_NPARAM, np.float32)
+        tb = np.zeros((16, self.T, 2), np.float32) if table else None
+        check(lib.ivr_surrogate_get(self.ctx.h, self.h, _ptr(mlp), _ptr(tb) if table else None, brick))
+        return mlp, tb
+
+    def set(self, mlp=None, table=None, brick=0):
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        check(lib.ivr_surrogate_set(self.ctx.h, self.h, None if m is None else _ptr(m),
+                                    None if t is None else _ptr(t), brick))
+
+    def render(self, vol, tf, desc, frame):
+        check(lib.ivr_surrogate_render(self.ctx.h, self.h, None if vol is None else vol.h, tf.h,
+                                       C.byref(desc), frame.h))
+
+    def evaluate(self, vol, tf, desc, threshold_db=40.0):
+        psnr = np.zeros(self.nbricks, np.float32)
+        nf = C.c_int()
+        check(lib.ivr_surrogate_evaluate(self.ctx.h, self.h, vol.h, tf.h, C.byref(desc), threshold_db,
+                                         _ptr(psnr), C.byref(nf)))
+        return psnr, nf.value
+
+    def set_fallback(self, flags):
+        f = np.ascontiguousarray(flags, np.uint8)
+        check(lib.ivr_surrogate_set_fallback(self.ctx.h, self.h, _ptr(f)))
+
+    def save(self, path):
+        check(lib.ivr_surrogate_save(self.ctx.h, self.h, path.encode()))
+
+    def load(self, path):
+        check(lib.ivr_surrogate_load(self.ctx.h, self.h, path.encode()))
+
+    def close(self):
+        if self.h:
+            lib.ivr_surrogate_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+def psnr(a, b):
+    """PSNR in dB of two RGBA8 images"""
+    d = a.astype(np.float64) - b.astype(np.float64)
+    mse = float(np.mean(d * d))
+    return 99.0 if mse == 0 else 10.0 * np.log10(255.0 ** 2 / mse)
+
+
+def ssim(a, b, win=7):
+    """mean SSIM (Wang et al. 2004) of the luminance of two RGBA8 images,
+    uniform window, C1=(0.01*255)^2, C2=(0.03*255)^2"""
+    def lum(x):
+        x = x.astype(np.float64)
+        return 0.299 * x[..., 0] + 0.587 * x[..., 1] + 0.114 * x[..., 2]
+    x, y = lum(a), lum(b)
+    k = win
+
+    def box(z):
+        c = np.cumsum(np.cumsum(np.pad(z, ((1, 0), (1, 0))), 0), 1)
+        return (c[k:, k:] - c[:-k, k:] - c[k:, :-k] + c[:-k, :-k]) / (k * k)
+    mx, my = box(x), box(y)
+    sxx = box(x * x) - mx * mx
+    syy = box(y * y) - my * my
+    sxy = box(x * y) - mx * my
+    c1, c2 = (0.01 * 255) ** 2, (0.03 * 255) ** 2
+    s = ((2 * mx * my + c1) * (2 * sxy + c2)) / ((mx * mx + my * my + c1) * (sxx + syy + c2))
+    return float(s.mean())

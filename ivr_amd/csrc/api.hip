@@ -262,6 +262,7 @@ void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const
     a.P = v->P;
     a.nbricks = v->nbricks;
     a.vox_data = v->data;
+    a.dtype_f16 = v->dtype == IVR_F16;
     a.boff = v->d_boff;
     a.cellempty = v->d_cellempty;
     a.lut = tf->d_lut;

@@ -96,6 +96,7 @@ struct RenderArgs {
     int bshift, bmask, P;
     int nbricks;
     const void *vox_data;
+    int dtype_f16;
     const unsigned long long *boff;
     const unsigned char *cellempty;
     int nc[3];

@@ -168,17 +168,23 @@ __global__ __launch_bounds__(256) void k_composite(const float4 *tiles, const Ti
 }
 
 /* macrocell transparency for this frame's LUT: a cell is empty when every
- * LUT entry its value range can index (widened by one entry each side for
- * interpolation rounding) has zero opacity */
+ * LUT entry its samples can index has zero opacity.  A sample is a chain of
+ * three lerps of the cell's voxels, so it lies in [min, max] up to a few
+ * ulps; the range is widened by 2^-20 relative (8 ulps) plus 2^-20 absolute
+ * before the monotone index map (int)(s * lutscale) - lutoffset. */
 __global__ void k_cells(const float2 *range, unsigned char *empty, size_t ncells, const float4 *lut,
                         int lutsize, float lutscale, int lutoffset, float alpha)
 {
     const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncells) return;
     const float2 r = range[c];
-    const float lo_s = fminf(fmaxf(r.x * lutscale, -1.0e9f), 1.0e9f);
-    const float hi_s = fminf(fmaxf(r.y * lutscale, -1.0e9f), 1.0e9f);
-    int lo = (int)lo_s - lutoffset - 1, hi = (int)hi_s - lutoffset + 1;
+    const float tol = 9.5367431640625e-07f; /* 2^-20 */
+    float lo_s = r.x * lutscale, hi_s = r.y * lutscale;
+    lo_s = lo_s - fabsf(lo_s) * tol - tol;
+    hi_s = hi_s + fabsf(hi_s) * tol + tol;
+    lo_s = fminf(fmaxf(lo_s, -1.0e9f), 1.0e9f);
+    hi_s = fminf(fmaxf(hi_s, -1.0e9f), 1.0e9f);
+    int lo = (int)lo_s - lutoffset, hi = (int)hi_s - lutoffset;
     lo = min(max(lo, 0), lutsize - 1);
     hi = min(max(hi, 0), lutsize - 1);
     unsigned char e = 1;

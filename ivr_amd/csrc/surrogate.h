@@ -1,0 +1,47 @@
+/*
+ * surrogate.h -- learned surrogate objects shared by surrogate.hip and the
+ * C-ABI (internal header).
+ */
+#ifndef IVR_SURROGATE_H
+#define IVR_SURROGATE_H
+
+#include "ivr_internal.h"
+
+#define SG_L 16  /* hash-grid levels            */
+#define SG_F 2   /* features per level          */
+#define SG_IN 32 /* encoding width = SG_L*SG_F  */
+#define SG_H 64  /* hidden width                */
+
+/* fp32 parameter block: W1[64][32] b1[64] W2[64][64] b2[64] W3[64] b3 */
+#define SG_P_W1 0
+#define SG_P_B1 (SG_P_W1 + SG_H * SG_IN)
+#define SG_P_W2 (SG_P_B1 + SG_H)
+#define SG_P_B2 (SG_P_W2 + SG_H * SG_H)
+#define SG_P_W3 (SG_P_B2 + SG_H)
+#define SG_P_B3 (SG_P_W3 + SG_H)
+#define SG_NPARAM (SG_P_B3 + 1)
+
+/* bf16 A-operand fragment sets (64 lanes x 8 bf16 each) */
+#define SG_FRAG_F1 0  /* W1, 4 row tiles                      */
+#define SG_FRAG_F2 4  /* W2, 4 row tiles x 2 k-steps, permuted */
+#define SG_FRAG_G2 12 /* W2^T, 4 x 2, permuted                 */
+#define SG_FRAG_G1 20 /* W1^T, 2 x 2, permuted                 */
+#define SG_NFRAG 24
+
+struct ivr_surrogate {
+    ivr_surrogate_params sp;
+    unsigned T = 0;
+    int nbricks = 0, B = 0;
+    int n[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+    int res[SG_L] = {};
+    unsigned dense_n1[SG_L] = {};
+    size_t ntable = 0;
+    float *tables = nullptr, *gtables = nullptr, *mt = nullptr, *vt = nullptr;
+    float *params = nullptr, *gparams = nullptr, *mp = nullptr, *vp = nullptr;
+    void *frag = nullptr;
+    float *d_loss = nullptr;
+    unsigned char *d_fallback = nullptr;
+    int step = 0;
+};
+
+#endif

@@ -1,0 +1,1066 @@
+/*
+ * surrogate.hip -- learned volume surrogate: per-brick multiresolution hash
+ * grids (T. Mueller et al., SIGGRAPH 2022) feeding one shared MLP
+ * 32 -> 64 -> 64 -> 1 (ReLU, linear output) whose two 64-wide layers run on
+ * gfx950 bf16 MFMA (v_mfma_f32_16x16x32_bf16) with fp32 accumulation.
+ * Trained with the kernels below (forward, backward, Adam) on samples drawn
+ * from the plain volume; rendered by k_render_sg in place of the 8-voxel
+ * trilinear fetch, brick by brick, with a per-brick fallback to the plain
+ * volume.  No reference counterpart: this is an addition beside the render
+ * path of the reference.
+ *
+ * Wave-level MLP.  A wavefront evaluates 64 samples.  Activations are kept
+ * transposed (features x samples): with the MFMA C layout (lane l holds rows
+ * 4*(l/16)+0..3 of column l%16 of a 16x16 tile) a layer's output tile lands
+ * in exactly the lanes that need it as the next layer's B operand (column =
+ * sample l%16), once the K order of the next product is permuted to the
+ * rows each lane group already holds.  The sum over K does not depend on its
+ * order, so the weights (A operand) are stored once per step with the same
+ * permutation and no activation ever moves between lanes.  Only the weight
+ * gradients (a product over samples) stage their operands through LDS.
+ *
+ * Encoding work is spread the same way: lane l computes levels 4*(l/16) ..
+ * 4*(l/16)+3 of the samples l%16, 16+l%16, 32+l%16, 48+l%16, which is the
+ * B-operand fragment of the first layer.
+ */
+#include <cstring>
+#include <cmath>
+#include <vector>
+
+#include "render_dev.h"
+#include "surrogate.h"
+
+namespace {
+
+using namespace ivr_dev;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f32x4 mfma(const bf16x8 &a, const bf16x8 &b, const f32x4 &c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+/* K order of the permuted products: slot j of lane group g in k-step ks */
+__host__ __device__ __forceinline__ int kperm(int ks, int g, int j) { return (2 * ks + (j >> 2)) * 16 + g * 4 + (j & 3); }
+
+/* ---------------------------------------------------------------- encoding */
+
+struct Enc {
+    const float *table; /* [nbricks][L][T][F] */
+    unsigned T;
+    int res[SG_L];
+    unsigned n1[SG_L];  /* N+1 for dense levels, 0 for hashed */
+};
+
+__device__ __forceinline__ unsigned corner_index(const Enc &e, int l, int x, int y, int z)
+{
+    const unsigned n1 = e.n1[l];
+    if (n1) return (unsigned)x + n1 * ((unsigned)y + n1 * (unsigned)z);
+    return ((unsigned)x * 1u ^ (unsigned)y * 2654435761u ^ (unsigned)z * 805459861u) & (e.T - 1u);
+}
+
+/* corner 0 and fractional weights of level l at local position x in [0,1] */
+__device__ __forceinline__ void level_cell(const Enc &e, int l, const float *x, int *v0, float *w)
+{
+    const float N = (float)e.res[l];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const float u = x[a] * N;
+        int v = (int)floorf(u);
+        v = min(max(v, 0), e.res[l] - 1);
+        v0[a] = v;
+        w[a] = u - (float)v;
+    }
+}
+
+__device__ __forceinline__ float2 encode_level(const Enc &e, int brick, int l, const float *x)
+{
+    int v0[3];
+    float w[3];
+    level_cell(e, l, x, v0, w);
+    const float2 *tb = (const float2 *)(e.table + ((size_t)brick * SG_L + l) * (size_t)e.T * SG_F);
+    float2 f = make_float2(0.0f, 0.0f);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
+        const float2 t = tb[corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz)];
+        f.x = fmaf(wc, t.x, f.x);
+        f.y = fmaf(wc, t.y, f.y);
+    }
+    return f;
+}
+
+__device__ __forceinline__ void scatter_level(const Enc &e, float *gtable, int brick, int l, const float *x,
+                                              float g0, float g1)
+{
+    int v0[3];
+    float w[3];
+    level_cell(e, l, x, v0, w);
+    float *tb = gtable + ((size_t)brick * SG_L + l) * (size_t)e.T * SG_F;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+        const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
+        const unsigned idx = corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz);
+        unsafeAtomicAdd(tb + (size_t)idx * 2, wc * g0);
+        unsafeAtomicAdd(tb + (size_t)idx * 2 + 1, wc * g1);
+    }
+}
+
+/* first-layer B fragments: lane l, sample nt*16+l%16, features 8g..8g+7 */
+struct SampleSet {
+    float x[4][3]; /* local positions of samples nt*16 + l%16 */
+    int b[4];      /* their bricks */
+};
+
+__device__ __forceinline__ void gather_samples(const float *x, int b, SampleSet &s)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const int src = nt * 16 + (lane & 15);
+        s.x[nt][0] = __shfl(x[0], src, 64);
+        s.x[nt][1] = __shfl(x[1], src, 64);
+        s.x[nt][2] = __shfl(x[2], src, 64);
+        s.b[nt] = __shfl(b, src, 64);
+    }
+}
+
+__device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, bf16x8 E[4])
+{
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float2 f = encode_level(e, s.b[nt], 4 * g + q, s.x[nt]);
+            E[nt][2 * q] = (__bf16)f.x;
+            E[nt][2 * q + 1] = (__bf16)f.y;
+        }
+    }
+}
+
+/* ---------------------------------------------------------------- MLP */
+
+struct Mlp {
+    const bf16x8 *frag; /* [SG_NFRAG][64] */
+    const float *p;     /* fp32 parameters */
+};
+
+/* forward of 64 samples: returns y for samples nt*16+l%16 in y[nt]; keeps
+ * the first-layer activations (bf16 fragments) and second-layer
+ * pre-activations for the backward pass */
+struct Fwd {
+    bf16x8 H[2][4];
+    f32x4 Z2[4][4];
+    float y[4];
+};
+
+__device__ __forceinline__ void mlp_forward(const Mlp &m, const bf16x8 E[4], Fwd &f)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    /* layer 1: Z1T[out][sample] = W1 E^T */
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a = m.frag[(SG_FRAG_F1 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = mfma(a, E[nt], zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = c[i] + m.p[SG_P_B1 + mt * 16 + g * 4 + i];
+                f.H[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
+            }
+        }
+    }
+    /* layer 2: Z2T = W2 H1T with the permuted K order */
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a0 = m.frag[(SG_FRAG_F2 + mt * 2 + 0) * 64 + lane];
+        const bf16x8 a1 = m.frag[(SG_FRAG_F2 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = mfma(a0, f.H[0][nt], zero);
+            c = mfma(a1, f.H[1][nt], c);
+#pragma unroll
+            for (int i = 0; i < 4; i++) c[i] += m.p[SG_P_B2 + mt * 16 + g * 4 + i];
+            f.Z2[mt][nt] = c;
+        }
+    }
+    /* layer 3 (fp32): y = W3 relu(Z2) + b3, reduced over the lane groups */
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        float acc = 0.0f;
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = f.Z2[mt][nt][i];
+                acc = fmaf(m.p[SG_P_W3 + mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc);
+            }
+        acc += __shfl_xor(acc, 16, 64);
+        acc += __shfl_xor(acc, 32, 64);
+        f.y[nt] = acc + m.p[SG_P_B3];
+    }
+}
+
+/* own sample's value from the y[nt] a lane holds (its sample is nt = l/16) */
+__device__ __forceinline__ float own(const float v[4])
+{
+    const int g = (threadIdx.x & 63) >> 4;
+    return g == 0 ? v[0] : (g == 1 ? v[1] : (g == 2 ? v[2] : v[3]));
+}
+
+/* ---------------------------------------------------------------- training */
+
+struct TrainArgs {
+    RenderArgs va;        /* plain volume (brick table, dims) for targets */
+    Enc enc;
+    Mlp mlp;
+    float *gtable;
+    float *gparam;        /* fp32 [SG_NPARAM] */
+    float *loss;          /* sum of squared errors */
+    long long nsamples;   /* samples of this step */
+    float inv_n;
+    unsigned seed, step;
+    const float *xs;      /* explicit samples: local x,y,z [n][3] (or null) */
+    const int *bs;        /* explicit bricks [n] */
+    const float *ts;      /* explicit targets [n] */
+    float *ys;            /* explicit: outputs [n] (or null) */
+};
+
+#define SG_LDS_STRIDE 72 /* bf16 per staged row: 64 samples + pad */
+
+/* Jarzynski & Olano, "Hash Functions for GPU Rendering" (JCGT 2020): pcg hash */
+__device__ __forceinline__ unsigned pcg_hash(unsigned v)
+{
+    const unsigned state = v * 747796405u + 2891336453u;
+    const unsigned word = ((state >> ((state >> 28u) + 4u)) ^ state) * 277803737u;
+    return (word >> 22u) ^ word;
+}
+
+__device__ __forceinline__ float field_at(const RenderArgs &a, const unsigned long long *boff, float gx,
+                                          float gy, float gz)
+{
+    gx = fminf(fmaxf(gx, 0.0f), a.nm1f[0]);
+    gy = fminf(fmaxf(gy, 0.0f), a.nm1f[1]);
+    gz = fminf(fmaxf(gz, 0.0f), a.nm1f[2]);
+    const int j = min((int)gx, a.n[0] - 2), i = min((int)gy, a.n[1] - 2), k = min((int)gz, a.n[2] - 2);
+    float v[8];
+    if (a.dtype_f16) fetch8<_Float16>(a, boff, j, i, k, v);
+    else fetch8<float>(a, boff, j, i, k, v);
+    return trilinear(v, gx - (float)j, gy - (float)i, gz - (float)k);
+}
+
+__global__ __launch_bounds__(128) void k_train(const TrainArgs A)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 lds[2][64 * SG_LDS_STRIDE * 3 + 32 * SG_LDS_STRIDE];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    __bf16 *sDZ2 = lds[wave], *sH1 = sDZ2 + 64 * SG_LDS_STRIDE, *sDZ1 = sH1 + 64 * SG_LDS_STRIDE,
+           *sE = sDZ1 + 64 * SG_LDS_STRIDE;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    f32x4 gW2[4][4], gW1[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+#pragma unroll
+        for (int b = 0; b < 4; b++) gW2[a][b] = zero;
+        gW1[a][0] = zero;
+        gW1[a][1] = zero;
+    }
+    float gb1[16], gb2[16], gw3[16], gb3 = 0.0f, lsum = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 16; q++) gb1[q] = gb2[q] = gw3[q] = 0.0f;
+
+    const long long ntiles = (A.nsamples + 63) / 64;
+    const long long ngroups = (ntiles + 1) / 2;
+    for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
+        const long long sid = (grp * 2 + wave) * 64 + lane;
+        const bool valid = sid < A.nsamples;
+        /* this lane's sample: brick, local position, target */
+        float x[3], t = 0.0f;
+        int b = 0;
+        if (A.xs) {
+            const long long s = valid ? sid : 0;
+            x[0] = A.xs[s * 3];
+            x[1] = A.xs[s * 3 + 1];
+            x[2] = A.xs[s * 3 + 2];
+            b = A.bs[s];
+            t = A.ts[s];
+        } else {
+            const RenderArgs &va = A.va;
+            b = (int)(sid % va.nbricks);
+            const int bc[3] = {b % va.nb[0], (b / va.nb[0]) % va.nb[1], b / (va.nb[0] * va.nb[1])};
+            float gv[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const unsigned h = pcg_hash(A.seed ^ pcg_hash(A.step * 3u + (unsigned)r) ^ pcg_hash((unsigned)sid * 3u + r));
+                const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
+                const int lo = bc[r] << va.bshift;
+                const float ext = (float)min(1 << va.bshift, (va.n[r] - 1) - lo);
+                gv[r] = (float)lo + u * ext;
+                x[r] = u * ext / (float)(1 << va.bshift);
+            }
+            t = field_at(va, va.boff, gv[0], gv[1], gv[2]);
+        }
+        SampleSet ss;
+        gather_samples(x, b, ss);
+        bf16x8 E[4];
+        encode_frags(A.enc, ss, E);
+        Fwd f;
+        mlp_forward(A.mlp, E, f);
+        /* loss gradient for the samples nt*16+l%16 */
+        float dy[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int src = nt * 16 + (lane & 15);
+            const float tn = __shfl(t, src, 64);
+            const bool vn = __shfl((int)valid, src, 64) != 0;
+            const float r = f.y[nt] - tn;
+            dy[nt] = vn ? 2.0f * r * A.inv_n : 0.0f;
+            if (g == 0 && vn) lsum = fmaf(r, r, lsum);
+            if (g == 0) gb3 += dy[nt];
+        }
+        if (A.ys && valid) A.ys[sid] = own(f.y);
+        /* output layer and second hidden layer: dZ2 = relu'(Z2) * W3 dy */
+        bf16x8 DZ[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++)
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int o = mt * 16 + g * 4 + i;
+                    const float z = f.Z2[mt][nt][i];
+                    gw3[mt * 4 + i] = fmaf(dy[nt], z > 0.0f ? z : 0.0f, gw3[mt * 4 + i]);
+                    const float dz = z > 0.0f ? A.mlp.p[SG_P_W3 + o] * dy[nt] : 0.0f;
+                    const __bf16 dzb = (__bf16)dz;
+                    gb2[mt * 4 + i] += (float)dzb;
+                    DZ[mt >> 1][nt][(mt & 1) * 4 + i] = dzb;
+                    sDZ2[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = dzb;
+                    sH1[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = f.H[mt >> 1][nt][(mt & 1) * 4 + i];
+                }
+        /* dH1T = W2^T dZ2T, masked by the first layer's ReLU */
+        bf16x8 D1[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 4; mt++) {
+            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G2 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G2 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                f32x4 c = mfma(a0, DZ[0][nt], zero);
+                c = mfma(a1, DZ[1][nt], c);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int o = mt * 16 + g * 4 + i;
+                    const float h1 = (float)f.H[mt >> 1][nt][(mt & 1) * 4 + i];
+                    const __bf16 d = (__bf16)(h1 > 0.0f ? c[i] : 0.0f);
+                    gb1[mt * 4 + i] += (float)d;
+                    D1[mt >> 1][nt][(mt & 1) * 4 + i] = d;
+                    sDZ1[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = d;
+                }
+            }
+        }
+        /* dET = W1^T dZ1T, scattered into the hash tables */
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                f32x4 c = mfma(a0, D1[0][nt], zero);
+                c = mfma(a1, D1[1][nt], c);
+                const bool vn = (nt * 16 + (lane & 15)) + ((grp * 2 + wave) * 64) < A.nsamples;
+                if (vn) {
+                    /* inputs mt*16 + 4g + i are levels mt*8 + 2g + {0,1} */
+                    scatter_level(A.enc, A.gtable, ss.b[nt], mt * 8 + 2 * g, ss.x[nt], c[0], c[1]);
+                    scatter_level(A.enc, A.gtable, ss.b[nt], mt * 8 + 2 * g + 1, ss.x[nt], c[2], c[3]);
+                }
+            }
+        }
+        /* encodings as ET[feature][sample] for the first layer's weight gradient */
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) sE[(g * 8 + j) * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = E[nt][j];
+        __syncthreads();
+        /* weight gradients, products over the 64 samples of the tile */
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int kc = ks * 32 + g * 8;
+#pragma unroll
+            for (int mo = 0; mo < 4; mo++) {
+                const bf16x8 a2 = *(const bf16x8 *)&sDZ2[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                const bf16x8 a1 = *(const bf16x8 *)&sDZ1[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+#pragma unroll
+                for (int ni = 0; ni < 4; ni++) {
+                    const bf16x8 bh = *(const bf16x8 *)&sH1[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                    gW2[mo][ni] = mfma(a2, bh, gW2[mo][ni]);
+                }
+#pragma unroll
+                for (int ni = 0; ni < 2; ni++) {
+                    const bf16x8 be = *(const bf16x8 *)&sE[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                    gW1[mo][ni] = mfma(a1, be, gW1[mo][ni]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    /* flush: weight tiles are unique per lane; per-output sums reduce over l%16 */
+#pragma unroll
+    for (int mo = 0; mo < 4; mo++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = mo * 16 + g * 4 + i;
+#pragma unroll
+            for (int ni = 0; ni < 4; ni++)
+                unsafeAtomicAdd(&A.gparam[SG_P_W2 + o * SG_H + ni * 16 + (lane & 15)], gW2[mo][ni][i]);
+#pragma unroll
+            for (int ni = 0; ni < 2; ni++)
+                unsafeAtomicAdd(&A.gparam[SG_P_W1 + o * SG_IN + ni * 16 + (lane & 15)], gW1[mo][ni][i]);
+        }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        float v1 = gb1[q], v2 = gb2[q], v3 = gw3[q];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) {
+            v1 += __shfl_xor(v1, off, 64);
+            v2 += __shfl_xor(v2, off, 64);
+            v3 += __shfl_xor(v3, off, 64);
+        }
+        if ((lane & 15) == 0) {
+            const int o = (q >> 2) * 16 + g * 4 + (q & 3);
+            unsafeAtomicAdd(&A.gparam[SG_P_B1 + o], v1);
+            unsafeAtomicAdd(&A.gparam[SG_P_B2 + o], v2);
+            unsafeAtomicAdd(&A.gparam[SG_P_W3 + o], v3);
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        gb3 += __shfl_xor(gb3, off, 64);
+        lsum += __shfl_xor(lsum, off, 64);
+    }
+    if (lane == 0) {
+        unsafeAtomicAdd(&A.gparam[SG_P_B3], gb3);
+        unsafeAtomicAdd(A.loss, lsum);
+    }
+}
+
+/* bf16 A-operand fragments of the weights for all four products */
+__global__ void k_frags(const float *p, bf16x8 *frag)
+{
+    const int lane = threadIdx.x & 63, set = blockIdx.x, g = lane >> 4, r = lane & 15;
+    bf16x8 v;
+    for (int j = 0; j < 8; j++) {
+        float w;
+        if (set < SG_FRAG_F2) { /* W1, natural K */
+            const int mt = set - SG_FRAG_F1;
+            w = p[SG_P_W1 + (mt * 16 + r) * SG_IN + g * 8 + j];
+        } else if (set < SG_FRAG_G2) { /* W2, permuted K */
+            const int mt = (set - SG_FRAG_F2) >> 1, ks = (set - SG_FRAG_F2) & 1;
+            w = p[SG_P_W2 + (mt * 16 + r) * SG_H + kperm(ks, g, j)];
+        } else if (set < SG_FRAG_G1) { /* W2^T, permuted K */
+            const int mt = (set - SG_FRAG_G2) >> 1, ks = (set - SG_FRAG_G2) & 1;
+            w = p[SG_P_W2 + kperm(ks, g, j) * SG_H + mt * 16 + r];
+        } else { /* W1^T, permuted K */
+            const int mt = (set - SG_FRAG_G1) >> 1, ks = (set - SG_FRAG_G1) & 1;
+            w = p[SG_P_W1 + kperm(ks, g, j) * SG_IN + mt * 16 + r];
+        }
+        v[j] = (__bf16)w;
+    }
+    frag[set * 64 + lane] = v;
+}
+
+/* Adam with bias correction (Kingma & Ba 2015) */
+__global__ void k_adam(float *p, const float *gr, float *m, float *v, size_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = gr[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+/* uniform init: tables U[-s, s], MLP weights U[-1/sqrt(fan_in), 1/sqrt(fan_in)], biases 0 */
+__global__ void k_init(float *p, size_t n, float scale, unsigned seed)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned h = pcg_hash(seed ^ pcg_hash((unsigned)i) ^ pcg_hash((unsigned)(i >> 32) + 0x9e3779b9u));
+    p[i] = ((float)(h >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * scale;
+}
+
+} // namespace
+
+/* ================================================================= host API */
+
+namespace {
+
+Enc make_enc(const ivr_surrogate *s)
+{
+    Enc e;
+    e.table = s->tables;
+    e.T = s->T;
+    for (int l = 0; l < SG_L; l++) {
+        e.res[l] = s->res[l];
+        e.n1[l] = s->dense_n1[l];
+    }
+    return e;
+}
+
+void volume_args(RenderArgs &a, const ivr_volume *v)
+{
+    memset(&a, 0, sizeof a);
+    for (int r = 0; r < 3; r++) {
+        a.n[r] = v->n[r];
+        a.nb[r] = v->nb[r];
+        a.nm1f[r] = (float)(v->n[r] - 1);
+    }
+    a.bshift = v->bshift;
+    a.bmask = v->B - 1;
+    a.P = v->P;
+    a.nbricks = v->nbricks;
+    a.vox_data = v->data;
+    a.boff = v->d_boff;
+    a.dtype_f16 = v->dtype == IVR_F16;
+}
+
+int refresh_frags(ivr_ctx *ctx, ivr_surrogate *s)
+{
+    hipLaunchKernelGGL(k_frags, dim3(SG_NFRAG), dim3(64), 0, ctx->stream, s->params, (bf16x8 *)s->frag);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+} // namespace
+
+int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogate_params *sp, ivr_surrogate **out)
+{
+    if (!ctx || !vol || !sp || !out) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (sp->log2_table < 10 || sp->log2_table > 24) return ivr_fail(IVR_ERR_ARG, "log2_table %d", sp->log2_table);
+    ivr_surrogate *s = new ivr_surrogate();
+    s->sp = *sp;
+    s->T = 1u << sp->log2_table;
+    s->nbricks = vol->nbricks;
+    for (int r = 0; r < 3; r++) {
+        s->nb[r] = vol->nb[r];
+        s->n[r] = vol->n[r];
+    }
+    s->B = vol->B;
+    for (int l = 0; l < SG_L; l++) {
+        s->res[l] = (int)std::floor(sp->base_res * std::pow((double)sp->growth, l));
+        const unsigned long long n1 = (unsigned long long)s->res[l] + 1;
+        s->dense_n1[l] = n1 * n1 * n1 <= s->T ? (unsigned)n1 : 0u;
+    }
+    s->ntable = (size_t)s->nbricks * SG_L * s->T * SG_F;
+    int rc;
+    if ((rc = ivr_dalloc(ctx, (void **)&s->tables, s->ntable * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->gtables, s->ntable * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->mt, s->ntable * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->vt, s->ntable * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->params, SG_NPARAM * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->gparams, SG_NPARAM * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->mp, SG_NPARAM * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->vp, SG_NPARAM * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->frag, SG_NFRAG * 64 * 16)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->d_loss, 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->d_fallback, IVR_MAX_BRICKS))) {
+        ivr_surrogate_destroy(ctx, s);
+        return rc;
+    }
+    const unsigned seed = (unsigned)(sp->seed ^ (sp->seed >> 32));
+    hipLaunchKernelGGL(k_init, dim3((unsigned)((s->ntable + 255) / 256)), dim3(256), 0, ctx->stream, s->tables,
+                       s->ntable, sp->init_scale, seed);
+    /* MLP: W1 fan-in 32, W2 fan-in 64, W3 fan-in 64; biases zero */
+    hipLaunchKernelGGL(k_init, dim3((SG_H * SG_IN + 255) / 256), dim3(256), 0, ctx->stream, s->params + SG_P_W1,
+                       (size_t)SG_H * SG_IN, 1.0f / std::sqrt((float)SG_IN), seed + 1u);
+    hipLaunchKernelGGL(k_init, dim3((SG_H * SG_H + 255) / 256), dim3(256), 0, ctx->stream, s->params + SG_P_W2,
+                       (size_t)SG_H * SG_H, 1.0f / std::sqrt((float)SG_H), seed + 2u);
+    hipLaunchKernelGGL(k_init, dim3(1), dim3(256), 0, ctx->stream, s->params + SG_P_W3, (size_t)SG_H,
+                       1.0f / std::sqrt((float)SG_H), seed + 3u);
+    IVR_HIP(hipGetLastError());
+    IVR_HIP(hipMemsetAsync(s->params + SG_P_B1, 0, SG_H * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->params + SG_P_B2, 0, SG_H * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->params + SG_P_B3, 0, 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->mt, 0, s->ntable * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->vt, 0, s->ntable * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->mp, 0, SG_NPARAM * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->vp, 0, SG_NPARAM * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->d_fallback, 0, IVR_MAX_BRICKS, ctx->stream));
+    if ((rc = refresh_frags(ctx, s))) return rc;
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    *out = s;
+    return IVR_OK;
+}
+
+int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
+{
+    if (!s) return IVR_OK;
+    if (ctx) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ivr_dfree(ctx, s->tables, s->ntable * 4);
+        ivr_dfree(ctx, s->gtables, s->ntable * 4);
+        ivr_dfree(ctx, s->mt, s->ntable * 4);
+        ivr_dfree(ctx, s->vt, s->ntable * 4);
+        ivr_dfree(ctx, s->params, SG_NPARAM * 4);
+        ivr_dfree(ctx, s->gparams, SG_NPARAM * 4);
+        ivr_dfree(ctx, s->mp, SG_NPARAM * 4);
+        ivr_dfree(ctx, s->vp, SG_NPARAM * 4);
+        ivr_dfree(ctx, s->frag, SG_NFRAG * 64 * 16);
+        ivr_dfree(ctx, s->d_loss, 4);
+        ivr_dfree(ctx, s->d_fallback, IVR_MAX_BRICKS);
+    }
+    delete s;
+    return IVR_OK;
+}
+
+namespace {
+
+int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long long n, const float *xs, const int *bs,
+                 const float *ts, float *ys)
+{
+    TrainArgs A;
+    memset(&A, 0, sizeof A);
+    if (vol) volume_args(A.va, vol);
+    A.enc = make_enc(s);
+    A.mlp.frag = (const bf16x8 *)s->frag;
+    A.mlp.p = s->params;
+    A.gtable = s->gtables;
+    A.gparam = s->gparams;
+    A.loss = s->d_loss;
+    A.nsamples = n;
+    A.inv_n = 1.0f / (float)n;
+    A.seed = (unsigned)(s->sp.seed * 0x9e3779b1ull);
+    A.step = (unsigned)s->step;
+    A.xs = xs;
+    A.bs = bs;
+    A.ts = ts;
+    A.ys = ys;
+    IVR_HIP(hipMemsetAsync(s->gtables, 0, s->ntable * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->gparams, 0, SG_NPARAM * 4, ctx->stream));
+    IVR_HIP(hipMemsetAsync(s->d_loss, 0, 4, ctx->stream));
+    const long long groups = ((n + 63) / 64 + 1) / 2;
+    const unsigned grid = (unsigned)std::min<long long>(groups, 4096);
+    hipLaunchKernelGGL(k_train, dim3(grid), dim3(128), 0, ctx->stream, A);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int adam_all(ivr_ctx *ctx, ivr_surrogate *s)
+{
+    s->step++;
+    const ivr_surrogate_params &p = s->sp;
+    const float c1 = (float)(1.0 - std::pow((double)p.beta1, s->step));
+    const float c2 = (float)(1.0 - std::pow((double)p.beta2, s->step));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((s->ntable + 255) / 256)), dim3(256), 0, ctx->stream, s->tables,
+                       s->gtables, s->mt, s->vt, s->ntable, p.lr, p.beta1, p.beta2, p.eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3((SG_NPARAM + 255) / 256), dim3(256), 0, ctx->stream, s->params, s->gparams,
+                       s->mp, s->vp, (size_t)SG_NPARAM, p.lr_mlp, p.beta1, p.beta2, p.eps, c1, c2);
+    IVR_HIP(hipGetLastError());
+    return refresh_frags(ctx, s);
+}
+
+} // namespace
+
+int ivr_surrogate_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, int steps, long long samples,
+                        float *loss_out)
+{
+    if (!ctx || !s || !vol) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (vol->nbricks != s->nbricks || vol->B != s->B) return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    if (samples < 64 || steps < 0) return ivr_fail(IVR_ERR_ARG, "samples %lld steps %d", samples, steps);
+    int rc;
+    for (int k = 0; k < steps; k++) {
+        if ((rc = launch_train(ctx, s, vol, samples, nullptr, nullptr, nullptr, nullptr))) return rc;
+        if ((rc = adam_all(ctx, s))) return rc;
+    }
+    if (loss_out) {
+        float l = 0.0f;
+        IVR_HIP(hipMemcpyAsync(&l, s->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+        IVR_HIP(hipStreamSynchronize(ctx->stream));
+        *loss_out = l / (float)samples;
+    }
+    return IVR_OK;
+}
+
+int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, const int *bricks,
+                       const float *targets, float *y_out, float *loss_out, float *gmlp_out,
+                       float *gtable_out, int gtable_brick)
+{
+    if (!ctx || !s || n < 1 || !xyz || !bricks || !targets) return ivr_fail(IVR_ERR_ARG, "bad argument");
+    for (int i = 0; i < n; i++)
+        if (bricks[i] < 0 || bricks[i] >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", bricks[i]);
+    float *d = nullptr;
+    const size_t bytes = (size_t)n * 4 * 6;
+    int rc = ivr_dalloc(ctx, (void **)&d, bytes);
+    if (rc) return rc;
+    float *dx = d, *dt = d + 3 * (size_t)n, *dy = d + 4 * (size_t)n;
+    int *db = (int *)(d + 5 * (size_t)n);
+    IVR_HIP(hipMemcpyAsync(dx, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    IVR_HIP(hipMemcpyAsync(dt, targets, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    IVR_HIP(hipMemcpyAsync(db, bricks, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = launch_train(ctx, s, nullptr, n, dx, db, dt, dy))) return rc;
+    if (y_out) IVR_HIP(hipMemcpyAsync(y_out, dy, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (gmlp_out) IVR_HIP(hipMemcpyAsync(gmlp_out, s->gparams, SG_NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (gtable_out) {
+        if (gtable_brick < 0 || gtable_brick >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "gtable brick");
+        const size_t per = (size_t)SG_L * s->T * SG_F;
+        IVR_HIP(hipMemcpyAsync(gtable_out, s->gtables + per * gtable_brick, per * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    }
+    float l = 0.0f;
+    IVR_HIP(hipMemcpyAsync(&l, s->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream));
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    if (loss_out) *loss_out = l / (float)n;
+    ivr_dfree(ctx, d, bytes);
+    return IVR_OK;
+}
+
+int ivr_surrogate_adam(ivr_ctx *ctx, ivr_surrogate *s)
+{
+    if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    int rc = adam_all(ctx, s);
+    if (rc) return rc;
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}
+
+int ivr_surrogate_get(ivr_ctx *ctx, const ivr_surrogate *s, float *mlp, float *table, int brick)
+{
+    if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (mlp) IVR_HIP(hipMemcpyAsync(mlp, s->params, SG_NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (table) {
+        if (brick < 0 || brick >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", brick);
+        const size_t per = (size_t)SG_L * s->T * SG_F;
+        IVR_HIP(hipMemcpyAsync(table, s->tables + per * brick, per * 4, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}
+
+int ivr_surrogate_set(ivr_ctx *ctx, ivr_surrogate *s, const float *mlp, const float *table, int brick)
+{
+    if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (mlp) IVR_HIP(hipMemcpyAsync(s->params, mlp, SG_NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
+    if (table) {
+        if (brick < 0 || brick >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", brick);
+        const size_t per = (size_t)SG_L * s->T * SG_F;
+        IVR_HIP(hipMemcpyAsync(s->tables + per * brick, table, per * 4, hipMemcpyHostToDevice, ctx->stream));
+    }
+    int rc = refresh_frags(ctx, s);
+    if (rc) return rc;
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}
+
+int ivr_surrogate_info(const ivr_surrogate *s, int *nbricks, unsigned long long *table_entries, int *step,
+                       int res[16])
+{
+    if (!s) return ivr_fail(IVR_ERR_ARG, "null surrogate");
+    if (nbricks) *nbricks = s->nbricks;
+    if (table_entries) *table_entries = s->T;
+    if (step) *step = s->step;
+    if (res)
+        for (int l = 0; l < SG_L; l++) res[l] = s->res[l];
+    return IVR_OK;
+}
+
+/* ================================================================= render */
+
+namespace {
+
+struct SgRender {
+    RenderArgs a;
+    Enc enc;
+    Mlp mlp;
+    const unsigned char *fallback; /* per brick: 1 = fetch the plain volume */
+    int only_brick;                /* >= 0: composite only that brick's samples */
+};
+
+/* Config-mode frame with the surrogate in place of the trilinear fetch.
+ * The march is wave-synchronous: every lane evaluates one sample per
+ * iteration (lanes whose ray ended feed a dummy sample) so the MLP's MFMAs
+ * see all 64 lanes; the loop ends when a ballot finds no live ray. */
+__global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
+{
+    extern __shared__ float4 smem[];
+    const RenderArgs &a = S.a;
+    float4 *s_lut = smem;
+    unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
+    stage(a, s_lut, s_boff);
+    int i, j;
+    if (!block_pixel(a, i, j)) return;
+    const bool inside = (i < a.H) && (j < a.W);
+    float d[3] = {0.0f, 0.0f, 1.0f};
+    float tin = 0.0f, dt = 1.0f;
+    int n = 0;
+    if (inside) {
+        cfg_dir(a, i, j, d);
+        float tout;
+        if (slab(a.dconst, a.o, d, tin, tout)) {
+            tin = fmaxf(tin, 0.0f);
+            const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            dt = (a.vox * a.step_vox) / dlen;
+            const float nf = 1.0f + (tout - tin) / dt + 0.5f;
+            n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
+        }
+    }
+    Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    unsigned cnt = 0, fetched = 0;
+    bool live = n > 0;
+    int k = 0;
+    const float invB = 1.0f / (float)(1 << a.bshift);
+    while (__ballot(live) != 0ull) {
+        float x[3] = {0.5f, 0.5f, 0.5f};
+        int b = 0;
+        Pos p;
+        if (live) {
+            p = cfg_pos(a, a.o, d, tin, dt, k);
+            const int bx = p.j >> a.bshift, by = p.i >> a.bshift, bz = p.k >> a.bshift;
+            b = bx + a.nb[0] * (by + a.nb[1] * bz);
+            x[0] = ((float)p.j + p.fx - (float)(bx << a.bshift)) * invB;
+            x[1] = ((float)p.i + p.fy - (float)(by << a.bshift)) * invB;
+            x[2] = ((float)p.k + p.fz - (float)(bz << a.bshift)) * invB;
+        }
+        SampleSet ss;
+        gather_samples(x, b, ss);
+        bf16x8 E[4];
+        encode_frags(S.enc, ss, E);
+        Fwd f;
+        mlp_forward(S.mlp, E, f);
+        const float yv = own(f.y);
+        if (live) {
+            if (S.only_brick < 0 || S.only_brick == b) {
+                float s = yv;
+                if (S.fallback[b]) {
+                    float v[8];
+                    if (a.dtype_f16) fetch8<_Float16>(a, s_boff, p.j, p.i, p.k, v);
+                    else fetch8<float>(a, s_boff, p.j, p.i, p.k, v);
+                    s = trilinear(v, p.fx, p.fy, p.fz);
+                    fetched++;
+                }
+                const float4 e = s_lut[lut_index(a, s)];
+                const float al = e.w * a.alpha;
+                const float w = (1.0f - acc.a) * al;
+                acc.r = fmaf(w, e.x, acc.r);
+                acc.g = fmaf(w, e.y, acc.g);
+                acc.b = fmaf(w, e.z, acc.b);
+                acc.a = acc.a + w;
+                cnt++;
+            }
+            k++;
+            if (acc.a >= a.ert || k >= n) live = false;
+        }
+    }
+    if (inside) {
+        const size_t px = (size_t)i * a.W + j;
+        a.out8[px] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
+        if (a.out32) a.out32[px] = make_float4(acc.r, acc.g, acc.b, acc.a);
+    }
+    count_add(&a.counters[0], cnt);
+    count_add(&a.counters[1], fetched);
+}
+
+/* sum of squared RGBA8 differences */
+__global__ void k_sqerr(const uchar4 *x, const uchar4 *y, size_t n, unsigned long long *out)
+{
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned v = 0;
+    if (p < n) {
+        const uchar4 u = x[p], w = y[p];
+        const int d0 = (int)u.x - w.x, d1 = (int)u.y - w.y, d2 = (int)u.z - w.z, d3 = (int)u.w - w.w;
+        v = (unsigned)(d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
+    }
+    count_add(out, v);
+}
+
+int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+              const ivr_frame_desc *d, ivr_frame *fr, const unsigned char *fallback, int only_brick)
+{
+    if (d->mode != IVR_MODE_CONFIG) return ivr_fail(IVR_ERR_UNSUPPORTED, "surrogate renders config mode only");
+    if (d->state.winwidth != fr->W || d->state.winheight != fr->H)
+        return ivr_fail(IVR_ERR_ARG, "state %dx%d does not match frame %dx%d", d->state.winwidth,
+                        d->state.winheight, fr->W, fr->H);
+    if (vol && (vol->nbricks != s->nbricks || vol->B != s->B))
+        return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    /* the geometry of the trained volume; a stand-in volume object carries it
+     * when no plain data is resident */
+    ivr_volume geo;
+    const ivr_volume *v = vol;
+    if (!v) {
+        for (int r = 0; r < 3; r++) {
+            geo.n[r] = s->n[r];
+            geo.nb[r] = s->nb[r];
+        }
+        geo.B = s->B;
+        geo.bshift = __builtin_ctz((unsigned)s->B);
+        geo.P = s->B + 2;
+        geo.nbricks = s->nbricks;
+        geo.d_boff = nullptr;
+        v = &geo;
+    }
+    ivr_cam cam;
+    ivr_cam_setup(&d->state, s->n[0], s->n[1], s->n[2], &cam);
+    SgRender S;
+    ivr_fill_args(S.a, cam, v, tf, d, fr);
+    if (!vol) S.a.nbricks = 0; /* no brick table to stage */
+    S.enc = make_enc(s);
+    S.mlp.frag = (const bf16x8 *)s->frag;
+    S.mlp.p = s->params;
+    S.fallback = fallback;
+    S.only_brick = only_brick;
+    S.a.nblk_x = (fr->W + 15) / 16;
+    S.a.nblk = S.a.nblk_x * ((fr->H + 15) / 16);
+    S.a.nblk_pad = (S.a.nblk + 7) & ~7;
+    IVR_HIP(hipMemsetAsync(fr->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    fr->timed = 1;
+    fr->composited = 0;
+    IVR_HIP(hipEventRecord(fr->e[0], ctx->stream));
+    const size_t lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
+    hipLaunchKernelGGL(k_render_sg, dim3(S.a.nblk_pad), dim3(IVR_BLOCK), lds, ctx->stream, S);
+    IVR_HIP(hipGetLastError());
+    IVR_HIP(hipEventRecord(fr->e[1], ctx->stream));
+    return IVR_OK;
+}
+
+} // namespace
+
+void ivr_surrogate_defaults(ivr_surrogate_params *sp)
+{
+    memset(sp, 0, sizeof *sp);
+    sp->log2_table = 19;
+    sp->base_res = 16;
+    sp->growth = 1.5f;
+    sp->lr = 1e-2f;
+    sp->lr_mlp = 1e-3f;
+    sp->beta1 = 0.9f;
+    sp->beta2 = 0.99f;
+    sp->eps = 1e-15f;
+    sp->init_scale = 1e-4f;
+    sp->seed = 4004;
+}
+
+int ivr_surrogate_render(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+                         const ivr_frame_desc *d, ivr_frame *fr)
+{
+    if (!ctx || !s || !tf || !d || !fr) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (!vol) {
+        std::vector<unsigned char> h(s->nbricks);
+        IVR_HIP(hipMemcpyAsync(h.data(), s->d_fallback, s->nbricks, hipMemcpyDeviceToHost, ctx->stream));
+        IVR_HIP(hipStreamSynchronize(ctx->stream));
+        for (unsigned char c : h)
+            if (c) return ivr_fail(IVR_ERR_ARG, "a brick falls back to the plain volume, which was not given");
+    }
+    return sg_launch(ctx, s, vol, tf, d, fr, s->d_fallback, -1);
+}
+
+int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned char *flags)
+{
+    if (!ctx || !s || !flags) return ivr_fail(IVR_ERR_ARG, "null argument");
+    IVR_HIP(hipMemcpyAsync(s->d_fallback, flags, s->nbricks, hipMemcpyHostToDevice, ctx->stream));
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}
+
+int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+                           const ivr_frame_desc *d, float threshold_db, float *psnr_out, int *nfallback)
+{
+    if (!ctx || !s || !vol || !tf || !d) return ivr_fail(IVR_ERR_ARG, "null argument");
+    const int W = d->state.winwidth, H = d->state.winheight;
+    ivr_frame *fp = nullptr, *fs = nullptr;
+    unsigned char *dflags = nullptr;
+    unsigned long long *dsum = nullptr;
+    int rc;
+    if ((rc = ivr_frame_create(ctx, W, H, 0, &fp)) || (rc = ivr_frame_create(ctx, W, H, 0, &fs)) ||
+        (rc = ivr_dalloc(ctx, (void **)&dflags, 2 * IVR_MAX_BRICKS)) ||
+        (rc = ivr_dalloc(ctx, (void **)&dsum, 8)))
+        return rc;
+    IVR_HIP(hipMemsetAsync(dflags, 1, IVR_MAX_BRICKS, ctx->stream));
+    IVR_HIP(hipMemsetAsync(dflags + IVR_MAX_BRICKS, 0, IVR_MAX_BRICKS, ctx->stream));
+    std::vector<unsigned char> flags(s->nbricks, 0);
+    int nf = 0;
+    const size_t np = (size_t)W * H;
+    for (int b = 0; b < s->nbricks; b++) {
+        if ((rc = sg_launch(ctx, s, vol, tf, d, fp, dflags, b))) return rc;
+        if ((rc = sg_launch(ctx, s, vol, tf, d, fs, dflags + IVR_MAX_BRICKS, b))) return rc;
+        IVR_HIP(hipMemsetAsync(dsum, 0, 8, ctx->stream));
+        hipLaunchKernelGGL(k_sqerr, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, fp->d_rgba8,
+                           fs->d_rgba8, np, dsum);
+        IVR_HIP(hipGetLastError());
+        unsigned long long sq = 0;
+        IVR_HIP(hipMemcpyAsync(&sq, dsum, 8, hipMemcpyDeviceToHost, ctx->stream));
+        IVR_HIP(hipStreamSynchronize(ctx->stream));
+        const double mse = (double)sq / (double)(np * 4);
+        const float psnr = mse > 0.0 ? (float)(10.0 * std::log10(255.0 * 255.0 / mse)) : 99.0f;
+        if (psnr_out) psnr_out[b] = psnr;
+        flags[b] = psnr < threshold_db;
+        nf += flags[b];
+    }
+    ivr_frame_destroy(ctx, fp);
+    ivr_frame_destroy(ctx, fs);
+    ivr_dfree(ctx, dflags, 2 * IVR_MAX_BRICKS);
+    ivr_dfree(ctx, dsum, 8);
+    if (nfallback) *nfallback = nf;
+    return ivr_surrogate_set_fallback(ctx, s, flags.data());
+}
+
+/* checkpoint: "IVRSG001", params struct, dims, step, then MLP (params, m, v)
+ * and tables (values, m, v) as little-endian float32 */
+int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
+{
+    if (!ctx || !s || !path) return ivr_fail(IVR_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "wb");
+    if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
+    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '1'};
+    int hdr[8] = {s->nbricks, s->B, s->n[0], s->n[1], s->n[2], s->step, (int)s->T, SG_NPARAM};
+    bool ok = fwrite(magic, 8, 1, f) == 1 && fwrite(&s->sp, sizeof s->sp, 1, f) == 1 &&
+              fwrite(hdr, sizeof hdr, 1, f) == 1;
+    std::vector<float> buf;
+    const float *dev[6] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt};
+    const size_t cnt[6] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable};
+    for (int k = 0; k < 6 && ok; k++) {
+        buf.resize(cnt[k]);
+        if (hipMemcpy(buf.data(), dev[k], cnt[k] * 4, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
+        else ok = fwrite(buf.data(), 4, cnt[k], f) == cnt[k];
+    }
+    fclose(f);
+    return ok ? IVR_OK : ivr_fail(IVR_ERR_IO, "writing %s failed", path);
+}
+
+int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
+{
+    if (!ctx || !s || !path) return ivr_fail(IVR_ERR_ARG, "null argument");
+    FILE *f = fopen(path, "rb");
+    if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
+    char magic[8];
+    ivr_surrogate_params sp;
+    int hdr[8];
+    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG001", 8) && fread(&sp, sizeof sp, 1, f) == 1 &&
+              fread(hdr, sizeof hdr, 1, f) == 1;
+    if (ok && (hdr[0] != s->nbricks || hdr[1] != s->B || hdr[6] != (int)s->T || hdr[7] != SG_NPARAM)) {
+        fclose(f);
+        return ivr_fail(IVR_ERR_ARG, "checkpoint %s does not match this surrogate", path);
+    }
+    std::vector<float> buf;
+    float *dev[6] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt};
+    const size_t cnt[6] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable};
+    for (int k = 0; k < 6 && ok; k++) {
+        buf.resize(cnt[k]);
+        ok = fread(buf.data(), 4, cnt[k], f) == cnt[k] &&
+             hipMemcpy(dev[k], buf.data(), cnt[k] * 4, hipMemcpyHostToDevice) == hipSuccess;
+    }
+    fclose(f);
+    if (!ok) return ivr_fail(IVR_ERR_IO, "reading %s failed", path);
+    s->sp = sp;
+    s->step = hdr[5];
+    int rc = refresh_frags(ctx, s);
+    if (rc) return rc;
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}

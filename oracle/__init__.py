@@ -119,3 +119,58 @@ def ref_lib():
     if not os.path.exists(REF_LIB):
         return None
     return C.CDLL(REF_LIB)
+
+
+# ---------------------------------------------------------------- surrogate oracle
+
+lib.ivros_level_res.restype = C.c_int
+lib.ivros_level_res.argtypes = [C.c_int, C.c_int, C.c_double]
+lib.ivros_index.restype = C.c_uint32
+lib.ivros_index.argtypes = [C.c_int, C.c_int, C.c_double, C.c_uint32, C.c_int, C.c_int, C.c_int]
+lib.ivros_encode.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+lib.ivros_encode_backward.argtypes = [C.c_uint32, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p]
+lib.ivros_mlp.restype = C.c_double
+lib.ivros_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                          C.c_void_p]
+lib.ivros_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
+                           C.c_float, C.c_float, C.c_int]
+
+SG_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def _v(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def sg_encode(table, T, x, nmin=16, growth=1.5):
+    """table float32 [16][T][2]; x float64 [3] -> enc float64 [32]"""
+    table = np.ascontiguousarray(table, np.float32)
+    x = np.ascontiguousarray(x, np.float64)
+    enc = np.zeros(32, np.float64)
+    lib.ivros_encode(_v(table), T, nmin, growth, _v(x), _v(enc))
+    return enc
+
+
+def sg_forward_backward(params, table, T, xyz, targets, bf16=True, nmin=16, growth=1.5):
+    """MLP + encoding forward/backward over points of one brick; returns
+    (y, loss, gparams, gtable) in double precision"""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    n = xyz.shape[0]
+    enc = np.stack([sg_encode(table, T, p, nmin, growth) for p in xyz])
+    params = np.ascontiguousarray(params, np.float32)
+    t = np.ascontiguousarray(targets, np.float32)
+    y = np.zeros(n, np.float64)
+    g = np.zeros(SG_NPARAM, np.float64)
+    denc = np.zeros((n, 32), np.float64)
+    loss = lib.ivros_mlp(_v(params), _v(enc), _v(t), n, int(bf16), _v(y), _v(g), _v(denc))
+    gt = np.zeros((16, T, 2), np.float64)
+    for k in range(n):
+        lib.ivros_encode_backward(T, nmin, growth, _v(np.ascontiguousarray(xyz[k])),
+                                  _v(np.ascontiguousarray(denc[k])), _v(gt))
+    return y, loss, g, gt
+
+
+def sg_adam(p, g, m, v, lr, b1, b2, eps, t):
+    p, g, m, v = (np.ascontiguousarray(a, np.float32).copy() for a in (p, g, m, v))
+    lib.ivros_adam(_v(p), _v(g), _v(m), _v(v), p.size, lr, b1, b2, eps, t)
+    return p, m, v

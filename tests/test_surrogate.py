@@ -1,0 +1,176 @@
+"""Learned surrogate: CPU oracle pins (finite differences, level split,
+Adam closed form) and, on the GPU, gradient parity of the MFMA training
+kernel with the bf16-emulating oracle, training gain, per-brick fallback and
+checkpoint round trip."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+import ivr_amd as ivr
+import oracle
+from ivr_amd import configs
+
+T_SMALL = 1 << 12  # small tables keep the CPU oracle fast
+
+
+def _params(rng):
+    p = np.zeros(oracle.SG_NPARAM, np.float32)
+    p[:64 * 32] = rng.uniform(-0.3, 0.3, 64 * 32)
+    p[64 * 32:64 * 33] = rng.uniform(-0.1, 0.1, 64)
+    o = 64 * 33
+    p[o:o + 4096] = rng.uniform(-0.2, 0.2, 4096)
+    p[o + 4096:o + 4160] = rng.uniform(-0.1, 0.1, 64)
+    p[o + 4160:o + 4224] = rng.uniform(-0.3, 0.3, 64)
+    p[-1] = 0.05
+    return p
+
+
+def test_level_resolutions_and_dense_split():
+    res = [oracle.lib.ivros_level_res(l, 16, 1.5) for l in range(16)]
+    assert res[:6] == [16, 24, 36, 54, 81, 121] and res[-1] == 7006  # floor(16 * 1.5**15) = floor(7006.04)
+    T = 1 << 19
+    # (N+1)^3 <= T: levels 0-3 dense (17^3 .. 55^3), level 4 (82^3) hashed
+    assert oracle.lib.ivros_index(3, 16, 1.5, T, 54, 54, 54) == 54 + 55 * (54 + 55 * 54)
+    assert oracle.lib.ivros_index(4, 16, 1.5, T, 81, 81, 81) < T
+
+
+def test_mlp_and_encoding_gradients_match_finite_differences():
+    rng = np.random.default_rng(1)
+    p = _params(rng)
+    table = rng.uniform(-0.5, 0.5, (16, T_SMALL, 2)).astype(np.float32)
+    xyz = rng.uniform(0.02, 0.98, (6, 3))
+    t = rng.uniform(0, 1, 6).astype(np.float32)
+    y, loss, g, gt = oracle.sg_forward_backward(p, table, T_SMALL, xyz, t, bf16=False)
+
+    def L(pp, tb):
+        return oracle.sg_forward_backward(pp, tb, T_SMALL, xyz, t, bf16=False)[1]
+    # parameters in float32 storage: central differences with h = 2^-10
+    h = 2.0 ** -10
+    for idx in list(rng.choice(64 * 32, 4)) + [64 * 32 + 3, 2112 + 7 * 64 + 5, 6208 + 9, 6272 + 11, 6336]:
+        pp, pm = p.copy(), p.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (L(pp, table) - L(pm, table)) / (2 * h)
+        assert abs(fd - g[idx]) <= 2e-3 * max(1.0, abs(fd)), (idx, fd, g[idx])
+    # table entries touched by the first point at a dense and a hashed level
+    for lvl in (1, 9):
+        nz = np.argwhere(np.abs(gt[lvl]) > 0)
+        e, f = nz[0]
+        tp, tm = table.copy(), table.copy()
+        tp[lvl, e, f] += h
+        tm[lvl, e, f] -= h
+        fd = (L(p, tp) - L(p, tm)) / (2 * h)
+        assert abs(fd - gt[lvl, e, f]) <= 2e-3 * max(1e-2, abs(fd)), (lvl, fd, gt[lvl, e, f])
+
+
+def test_adam_first_step_closed_form():
+    rng = np.random.default_rng(2)
+    p = rng.normal(size=100).astype(np.float32)
+    g = rng.normal(size=100).astype(np.float32)
+    z = np.zeros(100, np.float32)
+    p1, m1, v1 = oracle.sg_adam(p, g, z, z, 1e-2, 0.9, 0.99, 1e-15, 1)
+    # bias-corrected first step moves every parameter by lr * sign(g)
+    assert np.allclose(p1, p - 1e-2 * np.sign(g), atol=1e-6)
+    assert np.allclose(m1, 0.1 * g, rtol=1e-6) and np.allclose(v1, 0.01 * g * g, rtol=1e-5)
+
+
+# ---------------------------------------------------------------- GPU
+
+@pytest.fixture(scope="module")
+def small_vol(ctx):
+    gp = configs.gen_params(2)
+    host = ivr.generate_host(gp, 64, 64, 64)
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=32)
+    vol.upload(host)
+    yield vol, host
+    vol.close()
+
+
+@pytest.mark.gpu
+def test_gpu_gradients_match_oracle(ctx, small_vol):
+    vol, _ = small_vol
+    sg = ctx.surrogate(vol, log2_table=12)
+    rng = np.random.default_rng(3)
+    p = _params(rng)
+    tables = [rng.uniform(-0.5, 0.5, (16, T_SMALL, 2)).astype(np.float32) for _ in range(2)]
+    sg.set(mlp=p, table=tables[0], brick=0)
+    sg.set(table=tables[1], brick=1)
+    n = 200  # not a multiple of 64: exercises the padded tile
+    xyz = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    bricks = rng.integers(0, 2, n).astype(np.int32)
+    t = rng.uniform(0, 1, n).astype(np.float32)
+    y, loss, g, gt0 = sg.grad(xyz, bricks, t, table_brick=0)
+    # oracle: both bricks' points, MLP gradient summed; table gradient of brick 0
+    yo = np.zeros(n)
+    go = np.zeros(oracle.SG_NPARAM)
+    lo = 0.0
+    gto = None
+    for b in (0, 1):
+        sel = bricks == b
+        yb, lb, gb, gtb = oracle.sg_forward_backward(p, tables[b], T_SMALL, xyz[sel].astype(np.float64), t[sel])
+        yo[sel] = yb
+        nb = sel.sum()
+        go += gb * nb / n  # oracle normalises by its own batch size
+        lo += lb * nb / n
+        if b == 0:
+            gto = gtb * nb / n
+    print("max |y - y_oracle| = %.3e, loss %.6f vs %.6f" % (np.abs(y - yo).max(), loss, lo))
+    assert np.abs(y - yo).max() < 2e-3 * max(1.0, np.abs(yo).max())
+    assert abs(loss - lo) < 1e-3 * max(lo, 1e-3)
+    rel = np.abs(g - go).max() / np.abs(go).max()
+    relt = np.abs(gt0 - gto).max() / np.abs(gto).max()
+    print("MLP gradient max rel err %.3e, table gradient %.3e" % (rel, relt))
+    assert rel < 2e-2 and relt < 2e-2
+    # Adam on the GPU against the oracle's step from the same gradients
+    sg.adam()
+    p1, _ = sg.get(table=False)
+    z = np.zeros_like(p)
+    po, _, _ = oracle.sg_adam(p, g, z, z, sg.sp.lr_mlp, sg.sp.beta1, sg.sp.beta2, sg.sp.eps, 1)
+    assert np.abs(p1 - po).max() < 1e-6 + 1e-5 * np.abs(po).max()
+    sg.close()
+
+
+@pytest.mark.gpu
+def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
+    vol, host = small_vol
+    lut = configs.tf_lut(2)
+    tf = ctx.tf(lut)
+    st = configs.state(2, 160, 144)
+    d = ivr.config_desc(st)
+    fr = ctx.frame(160, 144)
+    ctx.render(vol, tf, d, fr)
+    plain = fr.download()
+    sg = ctx.surrogate(vol, log2_table=15)
+    sg.render(vol, tf, d, fr)
+    untrained = fr.download()
+    losses = [sg.train(vol, 40, 1 << 18) for _ in range(5)]
+    sg.render(vol, tf, d, fr)
+    trained = fr.download()
+    p0, p1 = ivr.psnr(untrained, plain), ivr.psnr(trained, plain)
+    print("losses", losses, "PSNR untrained %.2f dB trained %.2f dB SSIM %.4f" % (p0, p1, ivr.ssim(trained, plain)))
+    assert losses[-1] < losses[0]
+    assert p1 > p0 + 10.0
+    # all bricks forced to fall back: identical to the plain render
+    sg.set_fallback(np.ones(sg.nbricks, np.uint8))
+    sg.render(vol, tf, d, fr)
+    assert np.array_equal(fr.download(), plain)
+    # per-brick evaluation at an unreachable threshold flags every brick
+    psnr, nf = sg.evaluate(vol, tf, d, threshold_db=200.0)
+    assert nf == sg.nbricks and (psnr > 0).all()
+    psnr, nf = sg.evaluate(vol, tf, d, threshold_db=0.0)
+    assert nf == 0
+    # checkpoint round trip reproduces the render
+    sg.render(vol, tf, d, fr)
+    before = fr.download()
+    path = os.path.join(tempfile.mkdtemp(), "sg.ckpt")
+    sg.save(path)
+    sg2 = ctx.surrogate(vol, log2_table=15, seed=99)
+    sg2.load(path)
+    sg2.render(vol, tf, d, fr)
+    assert np.array_equal(fr.download(), before)
+    sg2.close()
+    sg.close()
+    fr.close()
+    tf.close()
