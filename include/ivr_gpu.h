@@ -188,6 +188,50 @@ int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned ch
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
 
+/* ---- steering-request batcher: continuous batching of render requests of
+ * any size, camera, transfer function and resident volume into persistent
+ * launches.  Requests are admitted into slots while a slot is free and the
+ * HBM budget (resident volumes + per-request frame, LUT and macrocell
+ * bytes) allows; up to two batches are in flight on two streams, so work
+ * admitted while a batch runs overlaps it on the device.  Replaces the
+ * one-frame-at-a-time loop of the render server (main.c:206-250). ---- */
+typedef struct ivr_batcher ivr_batcher;
+typedef struct ivr_request {
+    unsigned long long id;  /* caller's request id                              */
+    int volume;             /* index from ivr_batcher_add_volume                */
+    ivr_frame_desc desc;    /* config-mode frame (state carries camera and size) */
+    const float *lut;       /* RGBA LUT of lutsize <= 256 entries, copied at submit */
+    int lutsize;
+    unsigned char *rgba8;   /* host destination, winwidth*winheight*4 bytes      */
+    double t_submit;        /* ivr_now() clock, seconds                          */
+} ivr_request;
+typedef struct ivr_completion {
+    unsigned long long id;
+    int status;             /* 0 ok, negative: rejected (error code)             */
+    double t_submit, t_admit, t_done;
+    unsigned long long samples;
+} ivr_completion;
+typedef struct ivr_batcher_stats {
+    unsigned long long submitted, admitted, completed, rejected, batches;
+    unsigned long long bytes_in_use, budget;
+    int pending, in_flight;
+} ivr_batcher_stats;
+
+int ivr_batcher_create(ivr_ctx *ctx, unsigned long long hbm_budget, int max_slots, int max_width,
+                       int max_height, ivr_batcher **out);
+int ivr_batcher_destroy(ivr_batcher *b);
+/* make a resident volume servable; returns its index (>= 0) or an error */
+int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol);
+int ivr_batcher_submit(ivr_batcher *b, const ivr_request *req);
+/* retire finished batches, admit pending requests, launch a batch; with
+ * block != 0 waits for the oldest batch when nothing else can proceed;
+ * returns the number of completions ready to poll */
+int ivr_batcher_progress(ivr_batcher *b, int block);
+int ivr_batcher_poll(ivr_batcher *b, ivr_completion *out, int max);
+int ivr_batcher_stats_get(const ivr_batcher *b, ivr_batcher_stats *st);
+/* monotonic clock of the batcher timestamps, seconds */
+double ivr_now(void);
+
 /* ---- host-side helpers (no GPU) ---- */
 /* Defaults for a config-mode frame of the given state. */
 void ivr_desc_config(ivr_frame_desc *d);
@@ -204,6 +248,10 @@ void ivr_tf_build(uint64_t seed, float alpha_max, float jitter, uint64_t jitter_
 /* CPU evaluation of the synthetic generator, bit-identical to the device one
  * (same code, same operation order); out is x-fastest. */
 int ivr_generate_host(const ivr_gen_params *gp, int nx, int ny, int nz, float *out);
+/* STATE <-> its 184 wire bytes: the three ints in network byte order, the
+ * floats native little-endian (connection.c:167-200) */
+void ivr_state_pack(const ivr_state *st, unsigned char out[184]);
+void ivr_state_unpack(const unsigned char in[184], ivr_state *st);
 /* deterministic PRNG used by every generator: SplitMix64-seeded PCG32 */
 uint32_t ivr_rng_u32(uint64_t *state);
 

@@ -61,6 +61,22 @@ class SurrogateParams(C.Structure):
 
 SG_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1
 
+
+class Request(C.Structure):
+    _fields_ = [("id", C.c_ulonglong), ("volume", C.c_int), ("desc", FrameDesc), ("lut", C.c_void_p),
+                ("lutsize", C.c_int), ("rgba8", C.c_void_p), ("t_submit", C.c_double)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("id", C.c_ulonglong), ("status", C.c_int), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("samples", C.c_ulonglong)]
+
+
+class BatcherStats(C.Structure):
+    _fields_ = [("submitted", C.c_ulonglong), ("admitted", C.c_ulonglong), ("completed", C.c_ulonglong),
+                ("rejected", C.c_ulonglong), ("batches", C.c_ulonglong), ("bytes_in_use", C.c_ulonglong),
+                ("budget", C.c_ulonglong), ("pending", C.c_int), ("in_flight", C.c_int)]
+
 assert C.sizeof(State) == 184
 assert C.sizeof(GenParams) == 40
 assert C.sizeof(FrameDesc) == 220
@@ -115,6 +131,16 @@ def _load():
         "ivr_surrogate_set_fallback": (C.c_int, [P, P, P]),
         "ivr_surrogate_save": (C.c_int, [P, P, C.c_char_p]),
         "ivr_surrogate_load": (C.c_int, [P, P, C.c_char_p]),
+        "ivr_batcher_create": (C.c_int, [P, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
+        "ivr_batcher_destroy": (C.c_int, [P]),
+        "ivr_batcher_add_volume": (C.c_int, [P, P]),
+        "ivr_batcher_submit": (C.c_int, [P, C.POINTER(Request)]),
+        "ivr_batcher_progress": (C.c_int, [P, C.c_int]),
+        "ivr_batcher_poll": (C.c_int, [P, C.POINTER(Completion), C.c_int]),
+        "ivr_batcher_stats_get": (C.c_int, [P, C.POINTER(BatcherStats)]),
+        "ivr_now": (C.c_double, []),
+        "ivr_state_pack": (None, [C.POINTER(State), P]),
+        "ivr_state_unpack": (None, [P, C.POINTER(State)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -241,6 +267,9 @@ class Context:
 
     def surrogate(self, vol, **kw):
         return Surrogate(self, vol, **kw)
+
+    def batcher(self, budget_bytes, max_slots, max_width, max_height):
+        return Batcher(self, budget_bytes, max_slots, max_width, max_height)
 
 
 class Volume:
@@ -435,3 +464,76 @@ def ssim(a, b, win=7):
     c1, c2 = (0.01 * 255) ** 2, (0.03 * 255) ** 2
     s = ((2 * mx * my + c1) * (2 * sxy + c2)) / ((mx * mx + my * my + c1) * (sxx + syy + c2))
     return float(s.mean())
+
+
+def now():
+    return lib.ivr_now()
+
+
+def state_pack(st):
+    """STATE as the 184 bytes of the wire (connection.c:167-200)"""
+    out = np.zeros(184, np.uint8)
+    lib.ivr_state_pack(C.byref(st), _ptr(out))
+    return out.tobytes()
+
+
+def state_unpack(data):
+    buf = np.frombuffer(bytes(data), np.uint8).copy()
+    st = State()
+    lib.ivr_state_unpack(_ptr(buf), C.byref(st))
+    return st
+
+
+class Batcher:
+    """continuous batching of render requests (ivr_batcher_*)"""
+
+    def __init__(self, ctx, budget_bytes, max_slots, max_width, max_height):
+        self.ctx = ctx
+        h = C.c_void_p()
+        check(lib.ivr_batcher_create(ctx.h, budget_bytes, max_slots, max_width, max_height, C.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def add_volume(self, vol):
+        rc = lib.ivr_batcher_add_volume(self.h, vol.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def submit(self, rid, volume, desc, lut, out=None):
+        """queue a request; returns the host image array it will fill"""
+        W, H = desc.state.winwidth, desc.state.winheight
+        img = np.empty((H, W, 4), np.uint8) if out is None else out
+        lut = np.ascontiguousarray(lut, np.float32)
+        r = Request(id=rid, volume=volume, desc=desc, lut=lut.ctypes.data, lutsize=lut.shape[0],
+                    rgba8=img.ctypes.data, t_submit=now())
+        self._keep[rid] = (img, lut)
+        check(lib.ivr_batcher_submit(self.h, C.byref(r)))
+        return img
+
+    def progress(self, block=False):
+        rc = lib.ivr_batcher_progress(self.h, int(block))
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def poll(self, max_n=256):
+        buf = (Completion * max_n)()
+        n = lib.ivr_batcher_poll(self.h, buf, max_n)
+        out = []
+        for k in range(n):
+            c = buf[k]
+            img, _ = self._keep.pop(c.id, (None, None))
+            out.append(dict(id=c.id, status=c.status, t_submit=c.t_submit, t_admit=c.t_admit, t_done=c.t_done,
+                            samples=c.samples, image=img))
+        return out
+
+    def stats(self):
+        st = BatcherStats()
+        check(lib.ivr_batcher_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in BatcherStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib.ivr_batcher_destroy(self.h)
+            self.h = None

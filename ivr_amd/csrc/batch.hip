@@ -1,0 +1,442 @@
+/*
+ * batch.hip -- continuous batching of steering requests (ivr_batcher_*).
+ *
+ * The reference render server handles one STATE at a time: receive, render
+ * all pixels, composite, send (main.c:206-250).  Here requests of different
+ * sizes, cameras, transfer functions and resident volumes share persistent
+ * launches: k_render_batch's workgroups pull 16x16-pixel tiles from a
+ * device counter over the concatenated tile lists of every request in the
+ * batch, stage that request's arguments, LUT and brick table in LDS and
+ * render the tile with the same config-mode march as the single-frame
+ * kernel, so each image is bit-identical alone and in a mix.
+ *
+ * Host side: requests wait in a FIFO, are admitted into a slot while one is
+ * free and the HBM budget allows, and the admitted-but-unlaunched slots
+ * form the next batch.  Two batch buffers alternate on two streams, so a
+ * batch formed while another runs overlaps it on the device; finished
+ * batches are retired by polling their events.
+ */
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <vector>
+
+#include "render_dev.h"
+
+namespace {
+
+using namespace ivr_dev;
+
+#define BATCH_LUT 256
+
+__global__ __launch_bounds__(IVR_BLOCK) void k_render_batch(const RenderArgs *args, const int *prefix, int nslots,
+                                                            int total, unsigned *counter)
+{
+    __shared__ float4 s_lut[BATCH_LUT];
+    __shared__ unsigned long long s_boff[IVR_MAX_BRICKS / 8];
+    __shared__ RenderArgs s_a;
+    __shared__ int s_tile, s_slot;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            const int t = (int)atomicAdd(counter, 1u);
+            int sl = 0;
+            while (sl + 1 < nslots && prefix[sl + 1] <= t) sl++;
+            s_tile = t;
+            s_slot = sl;
+        }
+        __syncthreads();
+        const int tile = s_tile, slot = s_slot;
+        if (tile >= total) break;
+        {
+            const unsigned *src = (const unsigned *)&args[slot];
+            unsigned *dst = (unsigned *)&s_a;
+            for (int w = threadIdx.x; w < (int)(sizeof(RenderArgs) / 4); w += blockDim.x) dst[w] = src[w];
+        }
+        __syncthreads();
+        const RenderArgs &a = s_a;
+        for (int t = threadIdx.x; t < a.lutsize; t += blockDim.x) s_lut[t] = a.lut[t];
+        for (int t = threadIdx.x; t < a.nbricks; t += blockDim.x) s_boff[t] = a.boff[t];
+        __syncthreads();
+        const int lt = tile - prefix[slot];
+        const int bx = lt % a.nblk_x, by = lt / a.nblk_x;
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        const int j = bx * 16 + (w & 1) * 8 + (lane & 7);
+        const int i = by * 16 + (w >> 1) * 8 + (lane >> 3);
+        Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        unsigned cnt = 0, fetched = 0;
+        if (i < a.H && j < a.W) {
+            float d[3];
+            cfg_dir(a, i, j, d);
+            const bool skip = a.cellempty != nullptr;
+            if (a.dtype_f16) {
+                if (skip) march_config<_Float16, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                else march_config<_Float16, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+            } else {
+                if (skip) march_config<float, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                else march_config<float, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+            }
+            a.out8[(size_t)i * a.W + j] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
+        }
+        count_add(&a.counters[0], cnt);
+        count_add(&a.counters[1], fetched);
+        __syncthreads();
+    }
+}
+
+double now_s()
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Slot {
+    bool busy = false, launched = false;
+    ivr_request req;
+    std::vector<float> lut;
+    double t_admit = 0.0;
+    unsigned long long charge = 0;
+    uchar4 *d_img = nullptr;
+    float4 *d_lut = nullptr;
+    unsigned long long *d_ctr = nullptr;
+    unsigned char *d_cells = nullptr;
+    size_t cells_cap = 0;
+    unsigned char *h_img = nullptr;       /* pinned */
+    unsigned long long *h_ctr = nullptr;  /* pinned */
+    float4 *h_lut = nullptr;              /* pinned */
+};
+
+struct Batch {
+    bool active = false;
+    int stream = 0;
+    hipEvent_t done = nullptr;
+    std::vector<int> slots;
+    RenderArgs *d_args = nullptr, *h_args = nullptr;
+    int *d_prefix = nullptr, *h_prefix = nullptr;
+    unsigned *d_counter = nullptr;
+};
+
+} // namespace
+
+struct ivr_batcher {
+    ivr_ctx *ctx = nullptr;
+    unsigned long long budget = 0, used = 0;
+    int max_slots = 0, max_w = 0, max_h = 0;
+    hipStream_t streams[2] = {};
+    std::vector<const ivr_volume *> vols;
+    std::vector<Slot> slots;
+    std::deque<Slot> pending; /* requests with their LUT copies */
+    Batch batches[2];
+    int next_batch = 0;
+    std::deque<ivr_completion> done;
+    ivr_batcher_stats st = {};
+};
+
+double ivr_now(void) { return now_s(); }
+
+int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, int max_w, int max_h,
+                       ivr_batcher **out)
+{
+    if (!ctx || !out || max_slots < 1 || max_slots > 256 || max_w < 1 || max_h < 1)
+        return ivr_fail(IVR_ERR_ARG, "bad batcher arguments");
+    ivr_batcher *b = new ivr_batcher();
+    b->ctx = ctx;
+    b->budget = budget;
+    b->max_slots = max_slots;
+    b->max_w = max_w;
+    b->max_h = max_h;
+    b->st.budget = budget;
+    for (auto &s : b->streams) IVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    b->slots.resize(max_slots);
+    const size_t img = (size_t)max_w * max_h * 4;
+    int rc;
+    for (Slot &s : b->slots) {
+        if ((rc = ivr_dalloc(ctx, (void **)&s.d_img, img)) || (rc = ivr_dalloc(ctx, (void **)&s.d_lut, BATCH_LUT * 16)) ||
+            (rc = ivr_dalloc(ctx, (void **)&s.d_ctr, 16)))
+            return rc;
+        IVR_HIP(hipHostMalloc((void **)&s.h_img, img, hipHostMallocDefault));
+        IVR_HIP(hipHostMalloc((void **)&s.h_ctr, 16, hipHostMallocDefault));
+        IVR_HIP(hipHostMalloc((void **)&s.h_lut, BATCH_LUT * 16, hipHostMallocDefault));
+    }
+    for (int k = 0; k < 2; k++) {
+        Batch &bt = b->batches[k];
+        bt.stream = k;
+        IVR_HIP(hipEventCreateWithFlags(&bt.done, hipEventDisableTiming));
+        if ((rc = ivr_dalloc(ctx, (void **)&bt.d_args, sizeof(RenderArgs) * max_slots)) ||
+            (rc = ivr_dalloc(ctx, (void **)&bt.d_prefix, sizeof(int) * (max_slots + 1))) ||
+            (rc = ivr_dalloc(ctx, (void **)&bt.d_counter, sizeof(unsigned))))
+            return rc;
+        IVR_HIP(hipHostMalloc((void **)&bt.h_args, sizeof(RenderArgs) * max_slots, hipHostMallocDefault));
+        IVR_HIP(hipHostMalloc((void **)&bt.h_prefix, sizeof(int) * (max_slots + 1), hipHostMallocDefault));
+    }
+    *out = b;
+    return IVR_OK;
+}
+
+int ivr_batcher_destroy(ivr_batcher *b)
+{
+    if (!b) return IVR_OK;
+    for (auto &s : b->streams) (void)hipStreamSynchronize(s);
+    const size_t img = (size_t)b->max_w * b->max_h * 4;
+    for (Slot &s : b->slots) {
+        ivr_dfree(b->ctx, s.d_img, img);
+        ivr_dfree(b->ctx, s.d_lut, BATCH_LUT * 16);
+        ivr_dfree(b->ctx, s.d_ctr, 16);
+        ivr_dfree(b->ctx, s.d_cells, s.cells_cap);
+        if (s.h_img) (void)hipHostFree(s.h_img);
+        if (s.h_ctr) (void)hipHostFree(s.h_ctr);
+        if (s.h_lut) (void)hipHostFree(s.h_lut);
+    }
+    for (Batch &bt : b->batches) {
+        if (bt.done) (void)hipEventDestroy(bt.done);
+        ivr_dfree(b->ctx, bt.d_args, sizeof(RenderArgs) * b->max_slots);
+        ivr_dfree(b->ctx, bt.d_prefix, sizeof(int) * (b->max_slots + 1));
+        ivr_dfree(b->ctx, bt.d_counter, sizeof(unsigned));
+        if (bt.h_args) (void)hipHostFree(bt.h_args);
+        if (bt.h_prefix) (void)hipHostFree(bt.h_prefix);
+    }
+    for (auto &s : b->streams) (void)hipStreamDestroy(s);
+    delete b;
+    return IVR_OK;
+}
+
+int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol)
+{
+    if (!b || !vol) return ivr_fail(IVR_ERR_ARG, "null argument");
+    const unsigned long long bytes = vol->bytes + vol->ncells * 9ull;
+    if (b->used + bytes > b->budget)
+        return ivr_fail(IVR_ERR_BUDGET, "volume of %llu bytes exceeds the HBM budget", bytes);
+    if (vol->nbricks > IVR_MAX_BRICKS / 8) return ivr_fail(IVR_ERR_UNSUPPORTED, "too many bricks to batch");
+    b->used += bytes;
+    b->vols.push_back(vol);
+    b->st.bytes_in_use = b->used;
+    return (int)b->vols.size() - 1;
+}
+
+namespace {
+
+void reject(ivr_batcher *b, const ivr_request &r, int code)
+{
+    ivr_completion c = {};
+    c.id = r.id;
+    c.status = code;
+    c.t_submit = r.t_submit;
+    c.t_done = now_s();
+    b->done.push_back(c);
+    b->st.rejected++;
+}
+
+unsigned long long charge_of(const ivr_batcher *b, const ivr_request &r)
+{
+    const ivr_volume *v = b->vols[r.volume];
+    return (unsigned long long)r.desc.state.winwidth * r.desc.state.winheight * 4ull + BATCH_LUT * 16ull +
+           sizeof(RenderArgs) + (r.desc.skip ? v->ncells : 0ull);
+}
+
+int retire(ivr_batcher *b, Batch &bt, bool wait)
+{
+    if (!bt.active) return 0;
+    if (wait) IVR_HIP(hipEventSynchronize(bt.done));
+    else {
+        const hipError_t e = hipEventQuery(bt.done);
+        if (e == hipErrorNotReady) return 0;
+        if (e != hipSuccess) return ivr_fail(IVR_ERR_HIP, "batch event: %s", hipGetErrorString(e));
+    }
+    const double t = now_s();
+    for (int si : bt.slots) {
+        Slot &s = b->slots[si];
+        const size_t bytes = (size_t)s.req.desc.state.winwidth * s.req.desc.state.winheight * 4;
+        memcpy(s.req.rgba8, s.h_img, bytes);
+        ivr_completion c = {};
+        c.id = s.req.id;
+        c.status = 0;
+        c.t_submit = s.req.t_submit;
+        c.t_admit = s.t_admit;
+        c.t_done = t;
+        c.samples = s.h_ctr[0];
+        b->done.push_back(c);
+        b->used -= s.charge;
+        s.busy = false;
+        s.launched = false;
+        b->st.completed++;
+    }
+    bt.slots.clear();
+    bt.active = false;
+    b->st.in_flight--;
+    b->st.bytes_in_use = b->used;
+    return 0;
+}
+
+int admit(ivr_batcher *b)
+{
+    while (!b->pending.empty()) {
+        Slot &p = b->pending.front();
+        int free_slot = -1;
+        for (int k = 0; k < b->max_slots; k++)
+            if (!b->slots[k].busy) { free_slot = k; break; }
+        if (free_slot < 0) break;
+        const unsigned long long ch = charge_of(b, p.req);
+        if (b->used + ch > b->budget) break; /* FIFO: wait for memory to free up */
+        Slot &s = b->slots[free_slot];
+        const ivr_volume *v = b->vols[p.req.volume];
+        if (p.req.desc.skip && s.cells_cap < v->ncells) {
+            ivr_dfree(b->ctx, s.d_cells, s.cells_cap);
+            s.cells_cap = 0;
+            int rc = ivr_dalloc(b->ctx, (void **)&s.d_cells, v->ncells);
+            if (rc) return rc;
+            s.cells_cap = v->ncells;
+        }
+        s.req = p.req;
+        s.lut.swap(p.lut);
+        s.busy = true;
+        s.launched = false;
+        s.charge = ch;
+        s.t_admit = now_s();
+        b->used += ch;
+        b->pending.pop_front();
+        b->st.admitted++;
+        b->st.bytes_in_use = b->used;
+    }
+    b->st.pending = (int)b->pending.size();
+    return 0;
+}
+
+int launch(ivr_batcher *b)
+{
+    std::vector<int> ready;
+    for (int k = 0; k < b->max_slots; k++)
+        if (b->slots[k].busy && !b->slots[k].launched) ready.push_back(k);
+    if (ready.empty()) return 0;
+    Batch *bt = nullptr;
+    for (int q = 0; q < 2; q++) {
+        Batch &c = b->batches[(b->next_batch + q) & 1];
+        if (!c.active) { bt = &c; break; }
+    }
+    if (!bt) return 0;
+    b->next_batch = (bt->stream + 1) & 1;
+    /* FIFO order of admission */
+    std::sort(ready.begin(), ready.end(), [&](int x, int y) { return b->slots[x].t_admit < b->slots[y].t_admit; });
+    hipStream_t st = b->streams[bt->stream];
+    int total = 0;
+    bt->slots = ready;
+    for (size_t q = 0; q < ready.size(); q++) {
+        Slot &s = b->slots[ready[q]];
+        const ivr_volume *v = b->vols[s.req.volume];
+        const ivr_frame_desc &d = s.req.desc;
+        memcpy(s.h_lut, s.lut.data(), s.lut.size() * sizeof(float));
+        IVR_HIP(hipMemcpyAsync(s.d_lut, s.h_lut, s.lut.size() * sizeof(float), hipMemcpyHostToDevice, st));
+        ivr_tf tf;
+        tf.n = (int)(s.lut.size() / 4);
+        tf.d_lut = s.d_lut;
+        ivr_frame fr;
+        fr.W = d.state.winwidth;
+        fr.H = d.state.winheight;
+        fr.d_rgba8 = s.d_img;
+        fr.d_rgba32 = nullptr;
+        fr.d_counters = s.d_ctr;
+        ivr_cam cam;
+        ivr_cam_setup(&d.state, v->n[0], v->n[1], v->n[2], &cam);
+        RenderArgs &a = bt->h_args[q];
+        ivr_fill_args(a, cam, v, &tf, &d, &fr);
+        a.nblk_x = (fr.W + 15) / 16;
+        a.cellempty = d.skip ? s.d_cells : nullptr;
+        if (d.skip) {
+            int rc = ivr_launch_cells_on(st, v, a, s.d_cells);
+            if (rc) return rc;
+        }
+        IVR_HIP(hipMemsetAsync(s.d_ctr, 0, 16, st));
+        bt->h_prefix[q] = total;
+        total += a.nblk_x * ((fr.H + 15) / 16);
+        s.launched = true;
+    }
+    bt->h_prefix[ready.size()] = total;
+    IVR_HIP(hipMemcpyAsync(bt->d_args, bt->h_args, sizeof(RenderArgs) * ready.size(), hipMemcpyHostToDevice, st));
+    IVR_HIP(hipMemcpyAsync(bt->d_prefix, bt->h_prefix, sizeof(int) * (ready.size() + 1), hipMemcpyHostToDevice, st));
+    IVR_HIP(hipMemsetAsync(bt->d_counter, 0, sizeof(unsigned), st));
+    const int grid = std::min(total, 2048);
+    hipLaunchKernelGGL(k_render_batch, dim3(grid), dim3(IVR_BLOCK), 0, st, bt->d_args, bt->d_prefix,
+                       (int)ready.size(), total, bt->d_counter);
+    IVR_HIP(hipGetLastError());
+    for (int si : ready) {
+        Slot &s = b->slots[si];
+        const size_t bytes = (size_t)s.req.desc.state.winwidth * s.req.desc.state.winheight * 4;
+        IVR_HIP(hipMemcpyAsync(s.h_img, s.d_img, bytes, hipMemcpyDeviceToHost, st));
+        IVR_HIP(hipMemcpyAsync(s.h_ctr, s.d_ctr, 16, hipMemcpyDeviceToHost, st));
+    }
+    IVR_HIP(hipEventRecord(bt->done, st));
+    bt->active = true;
+    b->st.batches++;
+    b->st.in_flight++;
+    return 0;
+}
+
+} // namespace
+
+int ivr_batcher_submit(ivr_batcher *b, const ivr_request *r)
+{
+    if (!b || !r) return ivr_fail(IVR_ERR_ARG, "null argument");
+    b->st.submitted++;
+    const ivr_frame_desc &d = r->desc;
+    int code = 0;
+    if (r->volume < 0 || r->volume >= (int)b->vols.size()) code = IVR_ERR_ARG;
+    else if (d.mode != IVR_MODE_CONFIG || d.composite != IVR_FUSED || !(d.step_vox > 0.0f)) code = IVR_ERR_UNSUPPORTED;
+    else if (d.state.winwidth < 1 || d.state.winheight < 1 || d.state.winwidth > b->max_w ||
+             d.state.winheight > b->max_h)
+        code = IVR_ERR_ARG;
+    else if (!r->lut || r->lutsize < 1 || r->lutsize > BATCH_LUT || !r->rgba8) code = IVR_ERR_ARG;
+    else if (charge_of(b, *r) + b->used > b->budget && b->st.in_flight == 0 && b->pending.empty()) {
+        /* could never be admitted even on an idle server */
+        bool any_busy = false;
+        for (const Slot &s : b->slots) any_busy |= s.busy;
+        if (!any_busy) code = IVR_ERR_BUDGET;
+    }
+    if (code) {
+        reject(b, *r, code);
+        return IVR_OK;
+    }
+    Slot p;
+    p.req = *r;
+    p.lut.assign(r->lut, r->lut + 4 * r->lutsize);
+    b->pending.push_back(std::move(p));
+    b->st.pending = (int)b->pending.size();
+    return IVR_OK;
+}
+
+int ivr_batcher_progress(ivr_batcher *b, int block)
+{
+    if (!b) return ivr_fail(IVR_ERR_ARG, "null batcher");
+    int rc;
+    for (Batch &bt : b->batches)
+        if ((rc = retire(b, bt, false))) return rc;
+    if ((rc = admit(b)) || (rc = launch(b))) return rc;
+    if (block && b->done.empty()) {
+        /* wait for the oldest in-flight batch, then refill */
+        Batch *oldest = nullptr;
+        for (Batch &bt : b->batches)
+            if (bt.active && (!oldest || bt.stream == b->next_batch)) oldest = &bt;
+        if (oldest) {
+            if ((rc = retire(b, *oldest, true))) return rc;
+            if ((rc = admit(b)) || (rc = launch(b))) return rc;
+        }
+    }
+    return (int)b->done.size();
+}
+
+int ivr_batcher_poll(ivr_batcher *b, ivr_completion *out, int max)
+{
+    if (!b || !out) return ivr_fail(IVR_ERR_ARG, "null argument");
+    int n = 0;
+    while (n < max && !b->done.empty()) {
+        out[n++] = b->done.front();
+        b->done.pop_front();
+    }
+    return n;
+}
+
+int ivr_batcher_stats_get(const ivr_batcher *b, ivr_batcher_stats *st)
+{
+    if (!b || !st) return ivr_fail(IVR_ERR_ARG, "null argument");
+    *st = b->st;
+    st->pending = (int)b->pending.size();
+    return IVR_OK;
+}

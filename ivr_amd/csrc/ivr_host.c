@@ -233,6 +233,36 @@ void ivr_desc_config(ivr_frame_desc *d)
     d->composite = IVR_FUSED;
 }
 
+/* ---------------- wire format of STATE ---------------- */
+
+static uint32_t be32(uint32_t v)
+{
+    const unsigned char b[4] = {(unsigned char)(v >> 24), (unsigned char)(v >> 16), (unsigned char)(v >> 8),
+                                (unsigned char)v};
+    uint32_t r;
+    memcpy(&r, b, 4);
+    return r;
+}
+
+void ivr_state_pack(const ivr_state *st, unsigned char out[184])
+{
+    ivr_state w = *st;
+    w.finish = (int)be32((uint32_t)st->finish);
+    w.winwidth = (int)be32((uint32_t)st->winwidth);
+    w.winheight = (int)be32((uint32_t)st->winheight);
+    memcpy(out, &w, sizeof w);
+}
+
+void ivr_state_unpack(const unsigned char in[184], ivr_state *st)
+{
+    memcpy(st, in, sizeof *st);
+    uint32_t v[3];
+    memcpy(v, in, 12);
+    st->finish = (int)be32(v[0]);
+    st->winwidth = (int)be32(v[1]);
+    st->winheight = (int)be32(v[2]);
+}
+
 /* ---------------- transfer-function builder ---------------- */
 
 static float clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }

@@ -246,9 +246,14 @@ int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a)
 
 int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a)
 {
+    return ivr_launch_cells_on(ctx->stream, vol, a, vol->d_cellempty);
+}
+
+int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out)
+{
     const unsigned nb = (unsigned)((vol->ncells + 255) / 256);
-    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(256), 0, ctx->stream, vol->d_cellrange, vol->d_cellempty,
-                       vol->ncells, a.lut, a.lutsize, a.lutscale, a.lutoffset, a.alpha);
+    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(256), 0, stream, vol->d_cellrange, out, vol->ncells, a.lut,
+                       a.lutsize, a.lutscale, a.lutoffset, a.alpha);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }
