@@ -511,6 +511,10 @@ class Batcher:
         check(lib.ivr_batcher_submit(self.h, C.byref(r)))
         return img
 
+    def busy(self):
+        st = self.stats()
+        return st["pending"] > 0 or st["in_flight"] > 0
+
     def progress(self, block=False):
         rc = lib.ivr_batcher_progress(self.h, int(block))
         if rc < 0:

@@ -16,6 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libivr.so")
 COMPAT_LIB = os.path.join(HERE, "libivr_compat.so")
+SERVER = os.path.join(HERE, "ivrserver")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("IVR_ARCH", "gfx950")
 
@@ -43,7 +44,7 @@ def sources():
 
 
 def _split(files):
-    return ([f for f in files if not f.startswith("compat")],
+    return ([f for f in files if not f.startswith(("compat", "ivrserver"))],
             [f for f in files if f.startswith("compat")])
 
 
@@ -55,6 +56,8 @@ def build(verbose=False):
         jobs.append([HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, f), "-o",
                                            os.path.join(OUT, f + ".o")])
     for f in c:
+        if f.startswith("ivrserver"):
+            continue
         jobs.append(["gcc"] + C_FLAGS + ["-c", os.path.join(CSRC, f), "-o",
                                          os.path.join(OUT, f + ".o")])
     with ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
@@ -73,6 +76,11 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs +
          ["-L" + HERE, "-livr", "-Wl,-rpath,$ORIGIN", "-lm"])
     os.replace(tmp, COMPAT_LIB)
+    # steering server executable over the C-ABI
+    tmp = SERVER + ".tmp"
+    _run(["gcc"] + C_FLAGS + [os.path.join(CSRC, "ivrserver.c"), "-o", tmp, "-L" + HERE, "-livr",
+                              "-Wl,-rpath,$ORIGIN", "-lm"])
+    os.replace(tmp, SERVER)
     return LIB
 
 

@@ -121,6 +121,7 @@ struct Batch {
 struct ivr_batcher {
     ivr_ctx *ctx = nullptr;
     unsigned long long budget = 0, used = 0;
+    unsigned long long base = 0; /* bytes of the resident volumes */
     int max_slots = 0, max_w = 0, max_h = 0;
     hipStream_t streams[2] = {};
     std::vector<const ivr_volume *> vols;
@@ -208,6 +209,7 @@ int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol)
         return ivr_fail(IVR_ERR_BUDGET, "volume of %llu bytes exceeds the HBM budget", bytes);
     if (vol->nbricks > IVR_MAX_BRICKS / 8) return ivr_fail(IVR_ERR_UNSUPPORTED, "too many bricks to batch");
     b->used += bytes;
+    b->base += bytes;
     b->vols.push_back(vol);
     b->st.bytes_in_use = b->used;
     return (int)b->vols.size() - 1;
@@ -276,6 +278,11 @@ int admit(ivr_batcher *b)
             if (!b->slots[k].busy) { free_slot = k; break; }
         if (free_slot < 0) break;
         const unsigned long long ch = charge_of(b, p.req);
+        if (b->base + ch > b->budget) { /* could never fit: reject, keep the FIFO moving */
+            reject(b, p.req, IVR_ERR_BUDGET);
+            b->pending.pop_front();
+            continue;
+        }
         if (b->used + ch > b->budget) break; /* FIFO: wait for memory to free up */
         Slot &s = b->slots[free_slot];
         const ivr_volume *v = b->vols[p.req.volume];
@@ -384,12 +391,7 @@ int ivr_batcher_submit(ivr_batcher *b, const ivr_request *r)
              d.state.winheight > b->max_h)
         code = IVR_ERR_ARG;
     else if (!r->lut || r->lutsize < 1 || r->lutsize > BATCH_LUT || !r->rgba8) code = IVR_ERR_ARG;
-    else if (charge_of(b, *r) + b->used > b->budget && b->st.in_flight == 0 && b->pending.empty()) {
-        /* could never be admitted even on an idle server */
-        bool any_busy = false;
-        for (const Slot &s : b->slots) any_busy |= s.busy;
-        if (!any_busy) code = IVR_ERR_BUDGET;
-    }
+    else if (b->base + charge_of(b, *r) > b->budget) code = IVR_ERR_BUDGET; /* never fits */
     if (code) {
         reject(b, *r, code);
         return IVR_OK;

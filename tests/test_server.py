@@ -1,0 +1,155 @@
+"""The steering server (ivr_amd/ivrserver) over the reference frame
+protocol: concurrent clients in --listen mode (plain STATE and the versioned
+extension) and the drop-in --relay mode behind a relay that behaves like the
+reference's myserver (myserver.c:104-165).  Every returned frame equals the
+in-process render of the same request."""
+import os
+import socket
+import struct
+import subprocess
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import ivr_amd as ivr
+from ivr_amd import configs
+
+pytestmark = pytest.mark.gpu
+SERVER = os.path.join(os.path.dirname(ivr.LIB_PATH), "ivrserver")
+
+
+def free_port():
+    s = socket.socket()
+    s.bind(("127.0.0.1", 0))
+    p = s.getsockname()[1]
+    s.close()
+    return p
+
+
+def recv_all(sock, n):
+    buf = bytearray()
+    while len(buf) < n:
+        chunk = sock.recv(n - len(buf))
+        if not chunk:
+            raise ConnectionError("peer closed")
+        buf += chunk
+    return bytes(buf)
+
+
+def start_server(args):
+    p = subprocess.Popen([SERVER] + args, stderr=subprocess.PIPE, text=True)
+    t0 = time.time()
+    lines = []
+    while time.time() - t0 < 120:
+        line = p.stderr.readline()
+        lines.append(line)
+        if "ivrserver: ready" in line:
+            threading.Thread(target=lambda: [None for _ in p.stderr], daemon=True).start()
+            return p
+        if p.poll() is not None:
+            break
+    p.kill()
+    raise RuntimeError("server did not start: " + "".join(lines))
+
+
+@pytest.fixture(scope="module")
+def refs(ctx):
+    """the served volumes, generated in-process the same way"""
+    a = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    a.generate(configs.gen_params(1))
+    b = ctx.volume(96, 96, 96, ivr.IVR_F32, brick=64)
+    b.generate(configs.gen_params(2))
+    yield [a, b]
+    a.close()
+    b.close()
+
+
+def render_ref(ctx, vol, st, lut):
+    tf = ctx.tf(lut)
+    fr = ctx.frame(st.winwidth, st.winheight)
+    ctx.render(vol, tf, ivr.config_desc(st, skip=True), fr)
+    img = fr.download()
+    fr.close()
+    tf.close()
+    return img
+
+
+def ext_message(rid, volume, lut, st):
+    lut = np.ascontiguousarray(lut, np.float32)
+    return (b"IVRX" + struct.pack("<IQii", 1, rid, volume, lut.shape[0]) + lut.tobytes() + ivr.state_pack(st))
+
+
+def test_listen_mode_concurrent_clients(ctx, refs):
+    port = free_port()
+    srv = start_server(["--listen", str(port), "--volume", "gen:1:64", "--volume", "gen:2:96", "--slots", "6",
+                        "--max", "320", "320", "--budget-gb", "2"])
+    results, errors = {}, []
+
+    def client(cid):
+        try:
+            s = socket.create_connection(("127.0.0.1", port))
+            assert recv_all(s, 13) == b"Hello, world!"
+            s.sendall(b"Hello, server!")
+            rng = np.random.default_rng(cid)
+            for k in range(4):
+                w, h = [(96, 64), (160, 120), (320, 240), (200, 200)][(cid + k) % 4]
+                st = ivr.orbit_state(int(rng.integers(1 << 30)), w, h)
+                if k % 2 == 0:  # the plain reference protocol: volume 0, default LUT
+                    s.sendall(ivr.state_pack(st))
+                    img = np.frombuffer(recv_all(s, w * h * 4), np.uint8).reshape(h, w, 4)
+                    results[(cid, k)] = (0, st, configs.tf_lut(1), img)
+                else:  # extension: volume 1, jittered LUT, request id echoed
+                    lut = ivr.tf_build(2002, 0.05, 0.05, int(rng.integers(1 << 30)))
+                    rid = cid * 100 + k
+                    s.sendall(ext_message(rid, 1, lut, st))
+                    hdr = recv_all(s, 24)
+                    assert hdr[:4] == b"IVRR"
+                    got_id, status, gw, gh = struct.unpack("<Qiii", hdr[4:])
+                    assert (got_id, status, gw, gh) == (rid, 0, w, h)
+                    img = np.frombuffer(recv_all(s, w * h * 4), np.uint8).reshape(h, w, 4)
+                    results[(cid, k)] = (1, st, lut, img)
+            fin = ivr.State()
+            fin.finish = 1
+            s.sendall(ivr.state_pack(fin))
+            s.close()
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=client, args=(c,)) for c in range(3)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    srv.terminate()
+    srv.wait(timeout=30)
+    assert not errors, errors
+    assert len(results) == 12
+    for key, (vol, st, lut, img) in results.items():
+        assert np.array_equal(img, render_ref(ctx, refs[vol], st, lut)), key
+
+
+def test_relay_mode_is_a_dropin_render_server(ctx, refs):
+    """fake myserver: accept the GPU server, then a client; relay STATE and frames"""
+    port = free_port()
+    ls = socket.socket()
+    ls.setsockopt(socket.SOL_SOCKET, socket.SO_REUSEADDR, 1)
+    ls.bind(("127.0.0.1", port))
+    ls.listen(2)
+    srv = start_server(["--relay", "127.0.0.1:%d" % port, "--volume", "gen:1:64", "--max", "256", "256",
+                        "--budget-gb", "1"])
+    conn, _ = ls.accept()
+    conn.sendall(b"Hello, world!")
+    assert recv_all(conn, 14) == b"Hello, server!"
+    for seed in (3001, 3002):
+        st = ivr.orbit_state(seed, 128, 96)
+        conn.sendall(ivr.state_pack(st))
+        img = np.frombuffer(recv_all(conn, 128 * 96 * 4), np.uint8).reshape(96, 128, 4)
+        assert np.array_equal(img, render_ref(ctx, refs[0], st, configs.tf_lut(1)))
+    fin = ivr.State()
+    fin.finish = 1
+    conn.sendall(ivr.state_pack(fin))
+    assert srv.wait(timeout=60) == 0
+    conn.close()
+    ls.close()
