@@ -41,7 +41,12 @@ def main():
     ap.add_argument("--skip", type=int, default=-1, help="macrocell skipping (default per config)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (for scaling PMC byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build renders all bricks on a single MI355X "
               "(multi-GPU sharding is a design note, not a measured path); run with --gpus 1",
@@ -140,6 +145,24 @@ def main():
     ctx.close()
 
 
+def copy_calibration(mib):
+    """hipMemcpy device-to-device of a known size, 5 times: the copy kernel
+    reads and writes exactly mib MiB per dispatch"""
+    import ctypes as C
+    import ivr_amd  # noqa: F401  (loads the HIP runtime the library uses)
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    for _ in range(5):
+        assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
+
+
 def cpu_baseline(vol, st, lut, gpu_img, W, H, target_s):
     """Oracle config mode on a band of rows through the image centre."""
     import numpy as np
@@ -156,9 +179,13 @@ def cpu_baseline(vol, st, lut, gpu_img, W, H, target_s):
     rows = int(min(H, max(8, 8 * target_s / max(dt, 1e-3))))
     r0, r1 = max(0, mid - rows // 2), min(H, mid - rows // 2 + rows)
     log("cpu baseline: %d threads, rows %d..%d" % (threads, r0, r1))
-    t = time.perf_counter()
-    img32, n = oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)
-    dt = time.perf_counter() - t
+    oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)  # warm-up
+    times = []
+    while len(times) < 5 and sum(times) < target_s:
+        t = time.perf_counter()
+        img32, n = oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)
+        times.append(time.perf_counter() - t)
+    dt = float(np.median(times))
     ref8 = oracle.quantize(img32[r0:r1])
     d = int(np.abs(ref8.astype(int) - gpu_img[r0:r1].astype(int)).max())
     log("cpu baseline: %.2fs for %d samples; max |gpu-cpu| on those rows = %d/255" % (dt, n, d))
@@ -167,8 +194,8 @@ def cpu_baseline(vol, st, lut, gpu_img, W, H, target_s):
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": "rows %d-%d of the %dx%d frame (%d samples, %.1fs); max |gpu-cpu| %d/255"
-                  % (r0, r1, W, H, n, dt, d),
+        "sample": "rows %d-%d of the %dx%d frame, %d samples, median of %d timed runs (%.2fs each) after "
+                  "1 warm-up; max |gpu-cpu| %d/255" % (r0, r1, W, H, n, len(times), dt, d),
     }
 
 

@@ -12,6 +12,18 @@
 
 #include "ivr_internal.h"
 
+/* tuning knobs (results are identical for every setting) */
+#ifndef IVR_REUSE
+#define IVR_REUSE 1 /* keep the 8 voxels while consecutive samples share a cell */
+#endif
+#ifndef IVR_XCD_REMAP
+#define IVR_XCD_REMAP 1 /* give each XCD one contiguous band of workgroups */
+#endif
+#ifndef IVR_WAVE_W
+#define IVR_WAVE_W 8 /* pixels per wavefront row; IVR_WAVE_W * IVR_WAVE_H = 64 */
+#endif
+#define IVR_WAVE_H (64 / IVR_WAVE_W)
+
 namespace ivr_dev {
 
 typedef _Float16 h16;
@@ -162,6 +174,8 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
         }
     }
     int k = 0;
+    int cj = -1, ci = -1, ck = -1; /* cell of the voxels held in v */
+    float v[8];
     while (k < n) {
         const Pos p = cfg_pos(a, o, d, tin, dt, k);
         if (SKIP) {
@@ -169,12 +183,12 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
             if (a.cellempty[cell]) {
                 /* every sample of this cell is transparent: jump to the first
                  * sample past the cell exit, verified exactly on the grid */
-                const int cj[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
+                const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
                 float te = FLT_MAX;
 #pragma unroll
                 for (int r = 0; r < 3; r++) {
                     if (gd[r] != 0.0f) {
-                        const float face = (float)((cj[r] + (gd[r] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
+                        const float face = (float)((cc[r] + (gd[r] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
                         te = fminf(te, (face - go[r]) * rgd[r]);
                     }
                 }
@@ -192,8 +206,12 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
                 continue;
             }
         }
-        float v[8];
-        fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+        if (!IVR_REUSE || p.j != cj || p.i != ci || p.k != ck) {
+            fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+            cj = p.j;
+            ci = p.i;
+            ck = p.k;
+        }
         const float s = trilinear(v, p.fx, p.fy, p.fz);
         const float4 e = s_lut[lut_index(a, s)];
         const float al = e.w * a.alpha;
@@ -349,13 +367,23 @@ __device__ __forceinline__ void stage(const RenderArgs &a, float4 *s_lut, unsign
 __device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
 {
     const int b = blockIdx.x;
+#if IVR_XCD_REMAP
     const int per = a.nblk_pad >> 3;
     const int L = (b & 7) * per + (b >> 3);
+#else
+    const int L = b;
+#endif
     if (L >= a.nblk) return false;
     const int bx = L % a.nblk_x, by = L / a.nblk_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     j = bx * 16 + (w & 1) * 8 + (lane & 7);
     i = by * 16 + (w >> 1) * 8 + (lane >> 3);
+#if IVR_WAVE_W != 8
+    /* a wavefront covers IVR_WAVE_W x IVR_WAVE_H pixels; the workgroup still
+     * covers 16 x 16: waves stacked along the short side */
+    j = bx * 16 + (w % (16 / IVR_WAVE_W)) * IVR_WAVE_W + (lane % IVR_WAVE_W);
+    i = by * 16 + (w / (16 / IVR_WAVE_W)) * IVR_WAVE_H + (lane / IVR_WAVE_W);
+#endif
     return true;
 }
 
