@@ -70,6 +70,15 @@ int ivr_ctx_create(int device, ivr_ctx **out)
     IVR_HIP(hipSetDevice(device));
     ivr_ctx *c = new ivr_ctx();
     c->device = device;
+    if (const char *e = getenv("IVR_XCD_CHUNK")) {
+        if (!strcmp(e, "row")) c->xcd_chunk = IVR_CHUNK_ROW;
+        else if (!strcmp(e, "band")) c->xcd_chunk = IVR_CHUNK_BAND;
+        else c->xcd_chunk = atoi(e) > 0 ? atoi(e) : 0;
+    }
+    if (const char *e = getenv("IVR_WAVE_W")) {
+        const int w = atoi(e);
+        c->wave_w = (w == 4 || w == 16) ? w : 8;
+    }
     IVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) IVR_HIP(hipEventCreate(&ev));
     *out = c;

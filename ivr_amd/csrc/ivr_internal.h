@@ -28,8 +28,16 @@ int ivr_fail(int code, const char *fmt, ...);
                             __FILE__, __LINE__);                                               \
     } while (0)
 
+/* workgroup -> XCD chunking of the frame kernels (ivr_ctx::xcd_chunk) */
+#define IVR_CHUNK_ROW (-1)  /* one row of 16x16 blocks per chunk            */
+#define IVR_CHUNK_BAND (-2) /* one contiguous eighth of the blocks per XCD  */
+
 struct ivr_ctx {
     int device = 0;
+    /* launch-shape tuning, read from IVR_XCD_CHUNK / IVR_WAVE_W at creation
+     * (results are identical for every setting) */
+    int xcd_chunk = IVR_CHUNK_ROW;
+    int wave_w = 16;
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
     unsigned long long bytes = 0;
@@ -44,7 +52,7 @@ struct ivr_volume {
     int B = 0, bshift = 0, P = 0;      /* brick edge, log2, padded edge B+2   */
     int nb[3] = {0, 0, 0};              /* bricks per axis owning samples      */
     int nbricks = 0;
-    size_t stride = 0;                  /* elements per brick                  */
+    size_t stride = 0;                  /* elements per brick, brick_stride(P) */
     size_t bytes = 0;
     void *data = nullptr;               /* all bricks, Morton order            */
     std::vector<unsigned long long> h_boff; /* linear brick id -> element offset */
@@ -107,15 +115,44 @@ struct RenderArgs {
     float4 *out32;
     unsigned long long *counters;
     int nblk_x, nblk, nblk_pad;
+    int xcd_chunk;  /* blocks per XCD chunk, 0 = hardware order */
+    int wave_shift; /* log2 of a wavefront's pixel width          */
     /* sort-last tile pass */
     int tb[3];
     int trect[4]; /* x0, y0, w, h */
     float4 *tile;
 };
 
+/* Voxel order inside a brick: storage coordinates (x,y,z) in [0, P), P =
+ * B+2, are the brick's floor indices plus one with the ghost layer at 0 and
+ * B+1, in rows of P elements, x fastest; a brick holds P^3 elements rounded
+ * up to 64, the tail being padding (zero, never read).  A 4x4x4-blocked
+ * order halves the HBM bytes per frame but was measured slower for the
+ * render kernels on gfx950, whose cost is in the L1 gathers, not in HBM. */
+__host__ __device__ static inline size_t brick_stride(int P)
+{
+    return ((size_t)P * P * P + 63) & ~(size_t)63;
+}
+
+__host__ __device__ static inline unsigned brick_elem(int P, int x, int y, int z)
+{
+    return ((unsigned)z * (unsigned)P + (unsigned)y) * (unsigned)P + (unsigned)x;
+}
+
+/* inverse of brick_elem (padding elements decode to z >= P) */
+__host__ __device__ static inline void brick_coords(int P, unsigned e, int l[3])
+{
+    l[0] = (int)(e % P);
+    l[1] = (int)((e / P) % P);
+    l[2] = (int)(e / ((unsigned)P * P));
+}
+
 /* kernel arguments of one frame (api.hip); f may be null for per-ray calls */
 void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
                    const ivr_frame_desc *d, ivr_frame *f);
+
+/* grid of 16x16 pixel blocks over a w x h frame with the context's chunking */
+void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h);
 
 /* launchers (render.hip) */
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);

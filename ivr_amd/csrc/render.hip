@@ -36,9 +36,9 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
     extern __shared__ float4 smem[];
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
-    stage(a, s_lut, s_boff);
     int i, j;
-    if (!block_pixel(a, i, j)) return;
+    if (!block_pixel(a, i, j)) return; /* uniform per workgroup */
+    stage(a, s_lut, s_boff);
     const bool inside = (i < a.H) && (j < a.W);
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
     unsigned cnt = 0, fetched = 0;
@@ -76,9 +76,9 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
     extern __shared__ float4 smem[];
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
-    stage(a, s_lut, s_boff);
     int ti, tj;
-    if (!block_pixel(a, ti, tj)) return;
+    if (!block_pixel(a, ti, tj)) return; /* uniform per workgroup */
+    stage(a, s_lut, s_boff);
     const bool inside = (ti < a.trect[3]) && (tj < a.trect[2]);
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
     unsigned cnt = 0, fetched = 0;
@@ -205,18 +205,24 @@ void launch_one(const RenderArgs &a, hipStream_t s)
                        s, a);
 }
 
-void set_grid(RenderArgs &a, int w, int h)
+} // namespace
+
+void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h)
 {
     a.nblk_x = (w + 15) / 16;
     a.nblk = a.nblk_x * ((h + 15) / 16);
-    a.nblk_pad = (a.nblk + 7) & ~7;
+    int c = ctx->xcd_chunk;
+    if (c == IVR_CHUNK_ROW) c = a.nblk_x;
+    else if (c == IVR_CHUNK_BAND) c = (a.nblk + 7) / 8;
+    a.xcd_chunk = c > 0 ? c : 0;
+    const int pad = 8 * (c > 0 ? c : 1);
+    a.nblk_pad = (a.nblk + pad - 1) / pad * pad;
+    a.wave_shift = ctx->wave_w == 4 ? 2 : ctx->wave_w == 16 ? 4 : 3;
 }
-
-} // namespace
 
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip)
 {
-    set_grid(a, a.W, a.H);
+    ivr_set_grid(a, ctx, a.W, a.H);
     const bool f16 = vol->dtype == IVR_F16;
     hipStream_t s = ctx->stream;
     if (a.mode == IVR_MODE_COMPAT) {
@@ -235,7 +241,7 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
 
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a)
 {
-    set_grid(a, a.trect[2], a.trect[3]);
+    ivr_set_grid(a, ctx, a.trect[2], a.trect[3]);
     if (vol->dtype == IVR_F16)
         hipLaunchKernelGGL(k_tile<h16>, dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a), ctx->stream, a);
     else

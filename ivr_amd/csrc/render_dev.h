@@ -12,18 +12,6 @@
 
 #include "ivr_internal.h"
 
-/* tuning knobs (results are identical for every setting) */
-#ifndef IVR_REUSE
-#define IVR_REUSE 1 /* keep the 8 voxels while consecutive samples share a cell */
-#endif
-#ifndef IVR_XCD_REMAP
-#define IVR_XCD_REMAP 1 /* give each XCD one contiguous band of workgroups */
-#endif
-#ifndef IVR_WAVE_W
-#define IVR_WAVE_W 8 /* pixels per wavefront row; IVR_WAVE_W * IVR_WAVE_H = 64 */
-#endif
-#define IVR_WAVE_H (64 / IVR_WAVE_W)
-
 namespace ivr_dev {
 
 typedef _Float16 h16;
@@ -38,17 +26,17 @@ __device__ __forceinline__ void fetch8(const RenderArgs &a, const unsigned long 
                                        int i, int k, float v[8])
 {
     const int bid = (j >> a.bshift) + a.nb[0] * ((i >> a.bshift) + a.nb[1] * (k >> a.bshift));
-    const int P = a.P, PP = a.P * a.P;
     const VT *p = (const VT *)a.vox_data + s_boff[bid] +
-                  ((((k & a.bmask) + 1) * P + ((i & a.bmask) + 1)) * P + ((j & a.bmask) + 1));
+                  brick_elem(a.P, (j & a.bmask) + 1, (i & a.bmask) + 1, (k & a.bmask) + 1);
+    const int dy = a.P, dz = a.P * a.P;
     v[0] = ldv(p);
     v[1] = ldv(p + 1);
-    v[2] = ldv(p + P);
-    v[3] = ldv(p + P + 1);
-    v[4] = ldv(p + PP);
-    v[5] = ldv(p + PP + 1);
-    v[6] = ldv(p + PP + P);
-    v[7] = ldv(p + PP + P + 1);
+    v[2] = ldv(p + dy);
+    v[3] = ldv(p + dy + 1);
+    v[4] = ldv(p + dz);
+    v[5] = ldv(p + dz + 1);
+    v[6] = ldv(p + dz + dy);
+    v[7] = ldv(p + dz + dy + 1);
 }
 
 __device__ __forceinline__ float dot3n(float n0, float n1, float n2, const float *v)
@@ -148,6 +136,20 @@ struct Acc {
     float r, g, b, a;
 };
 
+/* one composited sample (front to back, config mode) */
+__device__ __forceinline__ void ftb_sample(const RenderArgs &a, const float4 *s_lut, const float v[8],
+                                           const Pos &p, Acc &acc)
+{
+    const float s = trilinear(v, p.fx, p.fy, p.fz);
+    const float4 e = s_lut[lut_index(a, s)];
+    const float al = e.w * a.alpha;
+    const float w = (1.0f - acc.a) * al;
+    acc.r = fmaf(w, e.x, acc.r);
+    acc.g = fmaf(w, e.y, acc.g);
+    acc.b = fmaf(w, e.z, acc.b);
+    acc.a = acc.a + w;
+}
+
 /* front-to-back march of one ray over the global sample grid */
 template <typename VT, bool SKIP>
 __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *s_lut,
@@ -163,7 +165,6 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
     const float nf = 1.0f + (tout - tin) / dt + 0.5f;
     /* guard for degenerate directions; never reached by a finite ray */
     const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
-    /* voxel-space ray for macrocell exits: g(t) ~ go + gd t */
     float go[3], rgd[3], gd[3];
     if (SKIP) {
 #pragma unroll
@@ -173,8 +174,10 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
             rgd[r] = 1.0f / gd[r];
         }
     }
+    /* One loop for skipping and sampling, so that within a wavefront the
+     * rays crossing empty cells and the rays sampling advance together. */
     int k = 0;
-    int cj = -1, ci = -1, ck = -1; /* cell of the voxels held in v */
+    int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     float v[8];
     while (k < n) {
         const Pos p = cfg_pos(a, o, d, tin, dt, k);
@@ -206,20 +209,13 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
                 continue;
             }
         }
-        if (!IVR_REUSE || p.j != cj || p.i != ci || p.k != ck) {
+        if (p.j != cj || p.i != ci || p.k != ck) { /* consecutive samples often share their 8 voxels */
             fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
             cj = p.j;
             ci = p.i;
             ck = p.k;
         }
-        const float s = trilinear(v, p.fx, p.fy, p.fz);
-        const float4 e = s_lut[lut_index(a, s)];
-        const float al = e.w * a.alpha;
-        const float w = (1.0f - acc.a) * al;
-        acc.r = fmaf(w, e.x, acc.r);
-        acc.g = fmaf(w, e.y, acc.g);
-        acc.b = fmaf(w, e.z, acc.b);
-        acc.a = acc.a + w;
+        ftb_sample(a, s_lut, v, p, acc);
         cnt++;
         fetched++;
         k++;
@@ -363,27 +359,26 @@ __device__ __forceinline__ void stage(const RenderArgs &a, float4 *s_lut, unsign
     __syncthreads();
 }
 
-/* XCD-aware workgroup -> 16x16 pixel block; returns false for padding */
+/* Workgroup -> 16x16 pixel block; returns false for padding.  Workgroups
+ * are dealt to the 8 XCDs round robin (XCD = blockIdx % 8).  With a chunk
+ * C > 0 each XCD renders runs of C consecutive blocks, so one L2 sees
+ * neighbouring blocks, while the runs stay interleaved over the frame so
+ * every XCD gets a share of the expensive centre.  Inside the block each
+ * of the four wavefronts covers 2^s x 2^(6-s) pixels, s = wave_shift. */
 __device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
 {
     const int b = blockIdx.x;
-#if IVR_XCD_REMAP
-    const int per = a.nblk_pad >> 3;
-    const int L = (b & 7) * per + (b >> 3);
-#else
-    const int L = b;
-#endif
+    int L = b;
+    if (a.xcd_chunk > 0) {
+        const int g = b >> 3, q = g / a.xcd_chunk, r = g - q * a.xcd_chunk;
+        L = ((q << 3) + (b & 7)) * a.xcd_chunk + r;
+    }
     if (L >= a.nblk) return false;
     const int bx = L % a.nblk_x, by = L / a.nblk_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    j = bx * 16 + (w & 1) * 8 + (lane & 7);
-    i = by * 16 + (w >> 1) * 8 + (lane >> 3);
-#if IVR_WAVE_W != 8
-    /* a wavefront covers IVR_WAVE_W x IVR_WAVE_H pixels; the workgroup still
-     * covers 16 x 16: waves stacked along the short side */
-    j = bx * 16 + (w % (16 / IVR_WAVE_W)) * IVR_WAVE_W + (lane % IVR_WAVE_W);
-    i = by * 16 + (w / (16 / IVR_WAVE_W)) * IVR_WAVE_H + (lane / IVR_WAVE_W);
-#endif
+    const int s = a.wave_shift, ww = 1 << s, per_row = 16 >> s;
+    j = bx * 16 + (w % per_row) * ww + (lane & (ww - 1));
+    i = by * 16 + (w / per_row) * (64 >> s) + (lane >> s);
     return true;
 }
 

@@ -793,9 +793,9 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
     const RenderArgs &a = S.a;
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
-    stage(a, s_lut, s_boff);
     int i, j;
-    if (!block_pixel(a, i, j)) return;
+    if (!block_pixel(a, i, j)) return; /* uniform per workgroup */
+    stage(a, s_lut, s_boff);
     const bool inside = (i < a.H) && (j < a.W);
     float d[3] = {0.0f, 0.0f, 1.0f};
     float tin = 0.0f, dt = 1.0f;
@@ -915,9 +915,7 @@ int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const
     S.mlp.p = s->params;
     S.fallback = fallback;
     S.only_brick = only_brick;
-    S.a.nblk_x = (fr->W + 15) / 16;
-    S.a.nblk = S.a.nblk_x * ((fr->H + 15) / 16);
-    S.a.nblk_pad = (S.a.nblk + 7) & ~7;
+    ivr_set_grid(S.a, ctx, fr->W, fr->H);
     IVR_HIP(hipMemsetAsync(fr->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
     fr->timed = 1;
     fr->composited = 0;

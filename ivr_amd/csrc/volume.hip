@@ -11,9 +11,10 @@
  * the brick that owns its floor index.
  *
  * Brick layout: brick (bx,by,bz) owns floor indices [b*B, b*B+B) per axis
- * (the last brick also owns the clamped tail); its storage is (B+2)^3
- * elements, x fastest, local index = global - b*B + 1, with out-of-volume
- * ghosts clamped to the edge voxel.  Bricks are stored in Morton order of
+ * (the last brick also owns the clamped tail); its storage coordinates are
+ * local = global - b*B + 1 in [0, B+2), with out-of-volume ghosts clamped to
+ * the edge voxel, in rows (brick_elem); padding elements are zero and
+ * never read.  Bricks are stored in Morton order of
  * their coordinates; d_boff maps the linear brick id to its element offset.
  */
 #include <algorithm>
@@ -28,18 +29,20 @@ namespace {
 
 typedef _Float16 h16;
 
+constexpr int MINMAX_PER = 16;
+
 struct BrickGeom {
     int n[3], nb[3], B, bshift, P;
     unsigned long long stride;
 };
 
-__device__ __forceinline__ void elem_coords(const BrickGeom &g, int bid, int e, int l[3], int gl[3],
+/* storage coordinates l of element e of brick bid (the inverse of
+ * brick_elem), its brick coordinates b and the clamped global voxel gl;
+ * returns false for a padding element */
+__device__ __forceinline__ bool elem_coords(const BrickGeom &g, int bid, int e, int l[3], int gl[3],
                                             int b[3])
 {
-    const int P = g.P;
-    l[0] = e % P;
-    l[1] = (e / P) % P;
-    l[2] = e / (P * P);
+    brick_coords(g.P, (unsigned)e, l);
     b[0] = bid % g.nb[0];
     b[1] = (bid / g.nb[0]) % g.nb[1];
     b[2] = bid / (g.nb[0] * g.nb[1]);
@@ -47,6 +50,7 @@ __device__ __forceinline__ void elem_coords(const BrickGeom &g, int bid, int e, 
         int x = b[r] * g.B + l[r] - 1;
         gl[r] = min(max(x, 0), g.n[r] - 1);
     }
+    return l[0] < g.P && l[1] < g.P && l[2] < g.P;
 }
 
 /* is element (brick b, local l) the one copy of its voxel that is counted */
@@ -68,10 +72,10 @@ __global__ void k_scatter(BrickGeom g, const unsigned long long *boff, const flo
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int bid = blockIdx.y;
-    if (e >= g.P * g.P * g.P) return;
+    if (e >= (int)g.stride) return;
     int l[3], gl[3], b[3];
-    elem_coords(g, bid, e, l, gl, b);
-    const float v = lin[((size_t)gl[2] * g.n[1] + gl[1]) * g.n[0] + gl[0]];
+    const bool real = elem_coords(g, bid, e, l, gl, b);
+    const float v = real ? lin[((size_t)gl[2] * g.n[1] + gl[1]) * g.n[0] + gl[0]] : 0.0f;
     dst[boff[bid] + e] = (VT)v;
 }
 
@@ -92,7 +96,7 @@ __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *
         l[r] = gl[r] - b[r] * g.B + 1;
     }
     const int bid = b[0] + g.nb[0] * (b[1] + g.nb[1] * b[2]);
-    lin[idx] = (float)src[boff[bid] + ((size_t)l[2] * g.P + l[1]) * g.P + l[0]];
+    lin[idx] = (float)src[boff[bid] + brick_elem(g.P, l[0], l[1], l[2])];
 }
 
 /* raw generator field into fp32 bricks */
@@ -100,26 +104,27 @@ __global__ void k_gen(BrickGeom g, const unsigned long long *boff, const GenTabl
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int bid = blockIdx.y;
-    if (e >= g.P * g.P * g.P) return;
+    if (e >= (int)g.stride) return;
     int l[3], gl[3], b[3];
-    elem_coords(g, bid, e, l, gl, b);
-    dst[boff[bid] + e] = gen_field(*tab, gl[0], gl[1], gl[2], g.n[0], g.n[1], g.n[2]);
+    const bool real = elem_coords(g, bid, e, l, gl, b);
+    dst[boff[bid] + e] = real ? gen_field(*tab, gl[0], gl[1], gl[2], g.n[0], g.n[1], g.n[2]) : 0.0f;
 }
 
 /* min/max key over counted voxels */
 __global__ void k_minmax(BrickGeom g, const unsigned long long *boff, const float *src,
                          unsigned *mm)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    /* MINMAX_PER elements per thread, then one atomic pair per workgroup */
+    __shared__ unsigned s_lo[4], s_hi[4];
     const int bid = blockIdx.y;
     unsigned lo = 0xffffffffu, hi = 0u;
-    if (e < g.P * g.P * g.P) {
+    for (int q = 0; q < MINMAX_PER; q++) {
+        const int e = (blockIdx.x * MINMAX_PER + q) * 256 + threadIdx.x;
         int l[3], gl[3], b[3];
-        elem_coords(g, bid, e, l, gl, b);
-        if (counted(g, l, b)) {
+        if (e < (int)g.stride && elem_coords(g, bid, e, l, gl, b) && counted(g, l, b)) {
             const unsigned k = gen_key(src[boff[bid] + e]);
-            lo = k;
-            hi = k;
+            lo = min(lo, k);
+            hi = max(hi, k);
         }
     }
     for (int off = 32; off > 0; off >>= 1) {
@@ -127,6 +132,13 @@ __global__ void k_minmax(BrickGeom g, const unsigned long long *boff, const floa
         hi = max(hi, (unsigned)__shfl_xor((int)hi, off, 64));
     }
     if ((threadIdx.x & 63) == 0) {
+        s_lo[threadIdx.x >> 6] = lo;
+        s_hi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+        hi = max(max(s_hi[0], s_hi[1]), max(s_hi[2], s_hi[3]));
         atomicMin(&mm[0], lo);
         atomicMax(&mm[1], hi);
     }
@@ -142,10 +154,9 @@ __global__ void k_radix(BrickGeom g, const unsigned long long *boff, const float
     __syncthreads();
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int bid = blockIdx.y;
-    if (e < g.P * g.P * g.P) {
+    if (e < (int)g.stride) {
         int l[3], gl[3], b[3];
-        elem_coords(g, bid, e, l, gl, b);
-        if (counted(g, l, b)) {
+        if (elem_coords(g, bid, e, l, gl, b) && counted(g, l, b)) {
             const unsigned k = gen_key(src[boff[bid] + e]);
             if ((k & pmask) == prefix) atomicAdd(&sh[(k >> shift) & dmask], 1u);
         }
@@ -185,8 +196,8 @@ __global__ void k_cellrange(BrickGeom g, const unsigned long long *boff, const V
                 const int b0 = min(x >> g.bshift, g.nb[0] - 1), b1 = min(y >> g.bshift, g.nb[1] - 1),
                           b2 = min(z >> g.bshift, g.nb[2] - 1);
                 const int bid = b0 + g.nb[0] * (b1 + g.nb[1] * b2);
-                const float v = (float)src[boff[bid] + ((size_t)(z - b2 * g.B + 1) * g.P +
-                                                        (y - b1 * g.B + 1)) * g.P + (x - b0 * g.B + 1)];
+                const float v = (float)src[boff[bid] + brick_elem(g.P, x - b0 * g.B + 1, y - b1 * g.B + 1,
+                                                                  z - b2 * g.B + 1)];
                 mn = fminf(mn, v);
                 mx = fmaxf(mx, v);
             }
@@ -292,8 +303,7 @@ int ivr_volume_create(ivr_ctx *ctx, int nx, int ny, int nz, int dtype, int brick
         delete v;
         return ivr_fail(IVR_ERR_ARG, "%d bricks exceed %d", v->nb[0] * v->nb[1] * v->nb[2], IVR_MAX_BRICKS);
     }
-    const size_t p3 = (size_t)v->P * v->P * v->P;
-    v->stride = (p3 + 63) & ~(size_t)63;
+    v->stride = brick_stride(v->P);
     const size_t esz = dtype == IVR_F16 ? 2 : 4;
     v->bytes = v->stride * v->nbricks * esz;
     /* Morton order of brick coordinates -> storage slot */
@@ -374,7 +384,7 @@ int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *v, const float *host)
     if (rc) return rc;
     IVR_HIP(hipMemcpyAsync(lin, host, total * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     const BrickGeom g = geom_of(v);
-    dim3 grid((unsigned)((g.P * g.P * g.P + 255) / 256), (unsigned)v->nbricks);
+    dim3 grid((unsigned)((g.stride + 255) / 256), (unsigned)v->nbricks);
     if (v->dtype == IVR_F16)
         hipLaunchKernelGGL(k_scatter<h16>, grid, dim3(256), 0, ctx->stream, g, v->d_boff, lin, (h16 *)v->data);
     else
@@ -427,12 +437,13 @@ int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *v, const ivr_gen_params *gp)
     else if ((rc = ivr_dalloc(ctx, (void **)&raw, total_e * sizeof(float)))) return rc;
     IVR_HIP(hipMemcpyAsync(d_tab, &tab, sizeof(GenTable), hipMemcpyHostToDevice, ctx->stream));
     const BrickGeom g = geom_of(v);
-    dim3 grid((unsigned)((g.P * g.P * g.P + 255) / 256), (unsigned)v->nbricks);
+    dim3 grid((unsigned)((g.stride + 255) / 256), (unsigned)v->nbricks);
     hipLaunchKernelGGL(k_gen, grid, dim3(256), 0, ctx->stream, g, v->d_boff, d_tab, raw);
     IVR_HIP(hipGetLastError());
     const unsigned init_mm[2] = {0xffffffffu, 0u};
     IVR_HIP(hipMemcpyAsync(d_aux, init_mm, sizeof init_mm, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(k_minmax, grid, dim3(256), 0, ctx->stream, g, v->d_boff, raw, d_aux);
+    const dim3 grid_mm((unsigned)((g.stride + 256 * MINMAX_PER - 1) / (256 * MINMAX_PER)), (unsigned)v->nbricks);
+    hipLaunchKernelGGL(k_minmax, grid_mm, dim3(256), 0, ctx->stream, g, v->d_boff, raw, d_aux);
     IVR_HIP(hipGetLastError());
     unsigned mm[2];
     IVR_HIP(hipMemcpyAsync(mm, d_aux, sizeof mm, hipMemcpyDeviceToHost, ctx->stream));

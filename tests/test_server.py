@@ -38,14 +38,15 @@ def recv_all(sock, n):
     return bytes(buf)
 
 
-def start_server(args):
+def start_server(args, until="ivrserver: ready"):
+    """start ivrserver and read its log up to the line containing `until`"""
     p = subprocess.Popen([SERVER] + args, stderr=subprocess.PIPE, text=True)
     t0 = time.time()
     lines = []
     while time.time() - t0 < 120:
         line = p.stderr.readline()
         lines.append(line)
-        if "ivrserver: ready" in line:
+        if until in line:
             threading.Thread(target=lambda: [None for _ in p.stderr], daemon=True).start()
             return p
         if p.poll() is not None:
@@ -137,9 +138,13 @@ def test_relay_mode_is_a_dropin_render_server(ctx, refs):
     ls.setsockopt(socket.SOL_SOCKET, socket.SO_REUSEADDR, 1)
     ls.bind(("127.0.0.1", port))
     ls.listen(2)
+    # the server connects to the relay once its volumes are resident and
+    # waits for the relay's greeting before it reports ready
     srv = start_server(["--relay", "127.0.0.1:%d" % port, "--volume", "gen:1:64", "--max", "256", "256",
-                        "--budget-gb", "1"])
+                        "--budget-gb", "1"], until="volume 0 ready")
+    ls.settimeout(60)
     conn, _ = ls.accept()
+    conn.settimeout(60)
     conn.sendall(b"Hello, world!")
     assert recv_all(conn, 14) == b"Hello, server!"
     for seed in (3001, 3002):
