@@ -1,18 +1,27 @@
 """Benchmark of the MI355X render path on one GPU.
 
-Workload (the headline configuration): a 512^3 fp32 volume (4-octave Perlin
-fBm + 32 Gaussian blobs, zeroed below its 40th percentile) stored as 2x2x2
-bricks of 256^3 with a one-voxel ghost layer, rendered to a 1024x1024 RGBA8
-frame with a 256-entry transfer function, 0.5-voxel steps and early ray
-termination at alpha >= 0.99 (config mode).  The volume, LUT and frame are
-resident in HBM before timing; a step is one full frame.
+Workload (the headline configuration, C2): a 512^3 fp32 volume (4-octave
+Perlin fBm + 32 Gaussian blobs, zeroed below its 40th percentile) stored as
+2x2x2 bricks of 256^3 with a one-voxel ghost layer, rendered to a 1024x1024
+RGBA8 frame with a 256-entry transfer function, 0.5-voxel steps, sort-last
+compositing of the per-brick partials (early ray termination at alpha >= 0.99
+inside each brick, "over" far to near).  Volume, LUT and frame are resident in
+HBM before timing; a step is one full frame.
 
-value = Gsamples/s = composited samples per frame / device time per frame.
-roofline: HBM bound; algorithmic bytes per frame = S * 32 B (8 fp32 voxels
-per trilinear sample, no reuse) + 4 B per output pixel, divided by the
-average device time of a frame (HIP events on the library's stream), against
-8 TB/s.  cpu_baseline: the oracle's config-mode CPU restatement (same
-samples, same arithmetic) timed on host cores over a band of rows.
+value = Gsamples/s = S / device time per frame, S = samples classified and
+composited (after macrocell skipping, up to early termination).
+roofline: HBM bound; algorithmic bytes per frame = S * 8 * sizeof(voxel) (the
+trilinear footprint with no reuse) + 4 B per output pixel (+ 32 B per tile
+pixel when tiles round-trip HBM), divided by the average device time of a
+frame (HIP events on the library's stream), against 8 TB/s.  traffic: the HBM
+bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE for the same
+command, scaled by the counters' reading for a copy of known size
+(profiles/traffic.json).
+cpu_baseline: the oracle (the CPU restatement of the same workload, same
+samples and arithmetic) on the host cores over a band of rows.
+reference_cpu: the reference ray.c renderer compiled from its sources
+(oracle/_ref), single core, against this library's compat mode on the GPU
+over the same volume, camera and LUT (bit-identical images).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
 """
@@ -20,6 +29,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -39,7 +49,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--skip", type=int, default=-1, help="macrocell skipping (default per config)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--composite", choices=["fused", "sort_last"], default=None,
+                    help="brick compositing (default per config)")
+    ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (for scaling PMC byte counters)")
@@ -61,6 +73,9 @@ def main():
     cfg = configs.CONFIGS[c]
     W, H = cfg["image"]
     skip = cfg["skip"] if args.skip < 0 else bool(args.skip)
+    names = {"fused": ivr.FUSED, "sort_last": ivr.SORT_LAST}
+    comp = cfg["composite"] if args.composite is None else names[args.composite]
+    comp_name = {v: k for k, v in names.items()}[comp]
     ctx = ivr.Context(0)
     log("generating", cfg["name"])
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
@@ -70,38 +85,49 @@ def main():
     lut = configs.tf_lut(c)
     tf = ctx.tf(lut)
     st = configs.state(c)
-    desc = ivr.config_desc(st, skip=skip)
+    desc = ivr.config_desc(st, skip=skip, composite=comp)
     fr = ctx.frame(W, H)
 
-    for _ in range(args.warmup):
-        ctx.render(vol, tf, desc, fr)
-    ctx.sync()
-    stats = fr.stats()
-    S = stats["samples"]
-    log("warmup done: samples/frame %d, fetched %d, last frame %.3f ms" % (S, stats["visited"], stats["render_ms"]))
+    def timed(d, steps, warmup):
+        for _ in range(warmup):
+            ctx.render(vol, tf, d, fr)
+        ctx.sync()
+        stats = fr.stats()
+        ctx.sync()
+        t_wall = time.perf_counter()
+        ctx.record(0)
+        for _ in range(steps):
+            ctx.render(vol, tf, d, fr)
+        ctx.record(1)
+        ctx.sync()
+        t_wall = time.perf_counter() - t_wall
+        return ctx.elapsed(0, 1) / steps, t_wall * 1e3 / steps, stats
 
-    ctx.sync()
-    t_wall = time.perf_counter()
-    ctx.record(0)
-    for _ in range(args.steps):
-        ctx.render(vol, tf, desc, fr)
-    ctx.record(1)
-    ctx.sync()
-    t_wall = time.perf_counter() - t_wall
-    dev_ms = ctx.elapsed(0, 1) / args.steps
-    ms_step = max(dev_ms, 1e-9)
-    gsps = S / (ms_step * 1e-3) / 1e9
-    vox_bytes = 2 if cfg["dtype"] == ivr.IVR_F16 else 4
-    fetched = stats["visited"]
-    bytes_alg = fetched * 8 * vox_bytes + 4 * W * H
-    achieved = bytes_alg / (ms_step * 1e-3)
-    log("timed %d frames: %.3f ms/frame device, %.3f ms/frame wall, %.1f Gsamples/s"
-        % (args.steps, dev_ms, t_wall * 1e3 / args.steps, gsps))
-
+    dev_ms, wall_ms, stats = timed(desc, args.steps, args.warmup)
+    S = stats["visited"]  # classified samples (equal to stats["samples"] without skipping)
+    gsps = S / (dev_ms * 1e-3) / 1e9
+    log("%s: %d samples classified per frame (%d incl. skipped); %.3f ms/frame device, %.3f wall, "
+        "%.1f Gsamples/s" % (comp_name, S, stats["samples"], dev_ms, wall_ms, gsps))
     img = fr.download()
-    cpu = None
+    vox_bytes = 2 if cfg["dtype"] == ivr.IVR_F16 else 4
+    bytes_alg = S * 8 * vox_bytes + 4 * W * H  # sort-last partials stay in registers: no tile bytes
+    achieved = bytes_alg / (dev_ms * 1e-3)
+    kernel = "k_render<1, %s, %s, %s>" % ("float" if vox_bytes == 4 else "_Float16", str(skip).lower(),
+                                          str(comp == ivr.SORT_LAST).lower())
+    traffic = measured_traffic(cfg["name"], comp_name, skip)
+
+    other = {}
+    alt = ivr.FUSED if comp != ivr.FUSED else ivr.SORT_LAST
+    a_ms, _, a_stats = timed(ivr.config_desc(st, skip=skip, composite=alt), args.steps, 2)
+    other[{v: k for k, v in names.items()}[alt]] = {
+        "ms_per_frame": round(a_ms, 4), "samples_per_frame": int(a_stats["visited"]),
+        "gsamples_s": round(a_stats["visited"] / a_ms / 1e6, 2)}
+    ctx.render(vol, tf, desc, fr)
+
+    cpu = ref = None
     if not args.no_cpu:
-        cpu = cpu_baseline(vol, st, lut, img, W, H, args.cpu_seconds)
+        cpu = cpu_baseline(vol, st, lut, img, W, H, comp, cfg["brick"], args.cpu_seconds)
+        ref = reference_cpu(ctx, vol, st, lut, W, H)
 
     out = {
         "metric": "Gsamples/s, ms/frame 512³→1024²; surrogate fps @PSNR≥40dB; p99 latency",
@@ -110,7 +136,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(dev_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -119,30 +145,46 @@ def main():
         "config": {
             "workload": cfg["name"],
             "image": "%dx%d" % (W, H),
+            "composite": comp_name,
+            "macrocell_skip": bool(skip),
             "samples_per_frame": int(S),
-            "fetched_per_frame": int(fetched),
+            "samples_incl_skipped": int(stats["samples"]),
             "step_voxels": configs.STEP_VOX,
             "ert": configs.ERT,
-            "macrocell_skip": bool(skip),
-            "frames_per_s": round(1000.0 / ms_step, 2),
+            "frames_per_s": round(1000.0 / dev_ms, 2),
+            "wall_ms_per_frame": round(wall_ms, 4),
+            "other_composite": other,
         },
         "roofline": {
             "bound": "hbm",
+            "kernel": kernel,
             "achieved": round(achieved / 1e9, 1),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK, 4),
-            "traffic": None,
+            "traffic": traffic,
         },
         "cpu_baseline": cpu,
+        "reference_cpu": ref,
     }
-    if cpu:
-        out["config"]["gpu_over_cpu"] = round(gsps / cpu["value"], 1) if cpu["value"] > 0 else None
+    if cpu and cpu["value"] > 0:
+        out["config"]["gpu_over_cpu"] = round(gsps / cpu["value"], 1)
     print(json.dumps(out), flush=True)
     fr.close()
     tf.close()
     vol.close()
     ctx.close()
+
+
+def measured_traffic(workload, comp_name, skip):
+    """HBM bytes per launch from the committed PMC summary, or None"""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    for e in json.load(open(path)).get("entries", []):
+        if e["workload"] == workload and e["composite"] == comp_name and e["skip"] == skip:
+            return int(e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024)
+    return None
 
 
 def copy_calibration(mib):
@@ -163,39 +205,163 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
 
 
-def cpu_baseline(vol, st, lut, gpu_img, W, H, target_s):
-    """Oracle config mode on a band of rows through the image centre."""
+def _oracle_frame(oracle, host, st, lut, comp, brick, rows, threads, f64=False):
+    """the oracle's render of rows [r0, r1) of the same workload: one pass
+    (fused) or per-brick tiles in visibility order + compositetiles"""
+    import ctypes as C
+    import numpy as np
+    import ivr_amd as ivr
+    v = oracle.view_of(st)
+    if comp == ivr.FUSED:
+        return oracle.render(host, v, lut, oracle.params(), rows=rows, nthreads=threads, f64=f64)
+    cam = np.zeros(23, np.float32)
+    rect = (C.c_int * 4)()
+    bmin, bmax = oracle.box(host.shape[2], host.shape[1], host.shape[0])
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
+    dims = (host.shape[2], host.shape[1], host.shape[0])
+    nb = [(dims[r] - 2) // brick + 1 for r in range(3)]
+    eb = [int(np.floor((cam[r] - bmin[r]) * (dims[r] / (bmax[r] - bmin[r])) / brick)) for r in range(3)]
+    ids = sorted([(x, y, z) for z in range(nb[2]) for y in range(nb[1]) for x in range(nb[0])],
+                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb[0] * (b[1] + nb[1] * b[2])))
+    tiles, s = [], 0
+    for b in ids:
+        t, k = oracle.render(host, v, lut, oracle.params(brick=brick, brick_id=b), rows=rows, nthreads=threads,
+                             f64=f64)
+        tiles.append(t)
+        s += k
+    if not f64:
+        return oracle.composite_tiles(np.stack(tiles)), s
+    d = np.zeros_like(tiles[0])  # compositetiles in double: far to near, d = s + (1 - s.a) d
+    for t in reversed(tiles):
+        d = t + (1.0 - t[..., 3:4]) * d
+    return d, s
+
+
+def cpu_baseline(vol, st, lut, gpu_img, W, H, comp, brick, target_s):
+    """The oracle on a band of rows through the image centre, sized to about
+    target_s seconds; median of up to 5 timed runs after a warm-up."""
     import numpy as np
     import oracle
     host = vol.download()
     threads = min(16, os.cpu_count() or 1)
-    v = oracle.view_of(st)
-    p = oracle.params()
     mid = H // 2
-    r0, r1 = mid - 4, mid + 4
     t = time.perf_counter()
-    _, n = oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)
+    _oracle_frame(oracle, host, st, lut, comp, brick, (mid - 4, mid + 4), threads)
     dt = time.perf_counter() - t
     rows = int(min(H, max(8, 8 * target_s / max(dt, 1e-3))))
     r0, r1 = max(0, mid - rows // 2), min(H, mid - rows // 2 + rows)
     log("cpu baseline: %d threads, rows %d..%d" % (threads, r0, r1))
-    oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)  # warm-up
+    _oracle_frame(oracle, host, st, lut, comp, brick, (r0, r1), threads)  # warm-up
     times = []
     while len(times) < 5 and sum(times) < target_s:
         t = time.perf_counter()
-        img32, n = oracle.render(host, v, lut, p, rows=(r0, r1), nthreads=threads)
+        img32, n = _oracle_frame(oracle, host, st, lut, comp, brick, (r0, r1), threads)
         times.append(time.perf_counter() - t)
     dt = float(np.median(times))
     ref8 = oracle.quantize(img32[r0:r1])
     d = int(np.abs(ref8.astype(int) - gpu_img[r0:r1].astype(int)).max())
-    log("cpu baseline: %.2fs for %d samples; max |gpu-cpu| on those rows = %d/255" % (dt, n, d))
+    # FP64 oracle on a narrower band: the error of the fp32 GPU result
+    f0, f1 = mid - 8, mid + 8
+    i64, _ = _oracle_frame(oracle, host, st, lut, comp, brick, (f0, f1), threads, f64=True)
+    d64 = int(np.abs(oracle.quantize(i64[f0:f1]).astype(int) - gpu_img[f0:f1].astype(int)).max())
+    log("cpu baseline: %.2fs for %d samples; max |gpu-cpu| on those rows = %d/255, vs fp64 %d/255"
+        % (dt, n, d, d64))
     return {
         "value": round(n / dt / 1e9, 4),
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
         "sample": "rows %d-%d of the %dx%d frame, %d samples, median of %d timed runs (%.2fs each) after "
-                  "1 warm-up; max |gpu-cpu| %d/255" % (r0, r1, W, H, n, len(times), dt, d),
+                  "1 warm-up; max |gpu-cpu| %d/255; max |gpu-fp64 oracle| %d/255 on rows %d-%d"
+                  % (r0, r1, W, H, n, len(times), dt, d, d64, f0, f1),
+    }
+
+
+class stdout_to_stderr:
+    """route file descriptor 1 to 2 (the reference prints progress to stdout)"""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        os.dup2(2, 1)
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+
+
+def reference_cpu(ctx, vol, st, lut, W, H):
+    """The reference's own ray.c/transferfuncs.c (compiled from its sources)
+    rendering one frame of the same volume (values x255), camera and LUT with
+    CHEM, single-threaded as the reference is; against the GPU compat mode
+    on identical input, whose image must be bit-identical."""
+    import ctypes as C
+    import numpy as np
+    import ivr_amd as ivr
+    import oracle
+    ref = oracle.ref_lib()
+    if ref is None:
+        return None
+    host = vol.download() * np.float32(255.0)
+    nz, ny, nx = host.shape
+    gv = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=vol.info()["brick"])
+    gv.upload(host)
+    tf = ctx.tf(lut)
+    fr = ctx.frame(W, H, keep_rgba32=True)
+    d = ivr.compat_desc(st, xfer=ivr.CHEM, maxlutindx=255)
+    ctx.render(gv, tf, d, fr)
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(5):
+        ctx.render(gv, tf, d, fr)
+    ctx.record(1)
+    ctx.sync()
+    g_ms = ctx.elapsed(0, 1) / 5
+    S = fr.stats()["samples"]
+    _, g32 = fr.download(rgba32=True)
+    fr.close()
+    tf.close()
+    gv.close()
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    fd, path = tempfile.mkstemp(suffix=".raw")
+    os.close(fd)
+    try:
+        host.tofile(path)
+        with stdout_to_stderr():
+            rc = ref.ivrref_init(path.encode(), nx, ny, nz, -1, 3, 8)
+    finally:
+        os.unlink(path)
+    del host
+    if rc != 0:
+        return None
+    lutc = np.ascontiguousarray(lut, np.float32)
+    ref.ivrref_set_lut(f(lutc), lutc.shape[0], 0, 255)
+    ref.ivrref_set_xfer(ivr.CHEM)
+    mvm = np.array(st.mvm, np.float32)
+    pm = np.array(st.pm, np.float32)
+    vv = np.array(st.vv, np.float32)
+    nc = np.array(st.newconst, np.float32)
+    out = np.zeros((H, W, 4), np.float32)
+    rect = (C.c_int * 4)()
+    with stdout_to_stderr():
+        t = time.perf_counter()
+        ref.ivrref_render(f(mvm), f(pm), f(vv), f(nc), C.c_float(st.alpha), W, H, f(out), rect)
+        c_s = time.perf_counter() - t
+    same = bool(np.array_equal(out.view(np.uint32), g32.view(np.uint32)))
+    log("reference ray.c: %.2fs for %d samples (1 core); GPU compat %.3f ms; images identical: %s"
+        % (c_s, S, g_ms, same))
+    return {
+        "value": round(S / c_s / 1e9, 5),
+        "unit": "Gsamples/s",
+        "cores": 1,
+        "kind": "reference",
+        "sample": "one %dx%d frame of the reference compat path (ray.c: 256-sample cap, 0.001 step, CHEM "
+                  "back-to-front) over the same volume x255, camera and LUT; %d samples" % (W, H, S),
+        "gpu_same_path_gsamples_s": round(S / g_ms / 1e6, 3),
+        "gpu_same_path_ms": round(g_ms, 4),
+        "gpu_over_reference": round((S / g_ms / 1e6) / (S / c_s / 1e9), 1),
+        "images_identical": same,
     }
 
 

@@ -53,9 +53,15 @@ extern "C" {
                              compositing with early ray termination */
 
 /* compositing of bricks */
-#define IVR_FUSED 0     /* each ray walks all bricks front to back in one launch */
-#define IVR_SORT_LAST 1 /* one partial RGBA32F tile per brick, then an "over" pass
-                           in visibility order (the reference's sort-last scheme) */
+#define IVR_FUSED 0           /* each ray walks all bricks front to back in one launch */
+#define IVR_SORT_LAST 1       /* the reference's sort-last scheme: one partial RGBA per
+                                 brick (front to back from zero, early termination
+                                 inside the brick), combined far to near with "over";
+                                 one launch, partials kept in registers */
+#define IVR_SORT_LAST_TILES 2 /* the same images with the partials materialised as
+                                 RGBA32F tiles in HBM (one launch per brick) and a
+                                 separate "over" pass: the exchange form of a
+                                 multi-GPU brick decomposition */
 
 /* synthetic volumes */
 #define IVR_GEN_BLOBS 0       /* sum of Gaussian blobs, min-max normalised to [0,1] */
@@ -89,14 +95,14 @@ typedef struct ivr_frame_desc {
     int lutoffset;
     int maxlutindx;    /* SEISMIC2 upper clamp                                 */
     int skip;          /* config: macrocell empty-space skipping on/off        */
-    int composite;     /* IVR_FUSED or IVR_SORT_LAST                           */
+    int composite;     /* IVR_FUSED, IVR_SORT_LAST or IVR_SORT_LAST_TILES       */
 } ivr_frame_desc;
 
 typedef struct ivr_frame_stats {
     unsigned long long samples; /* samples classified and composited           */
     unsigned long long visited; /* samples whose voxels were fetched (after skip) */
     float render_ms;            /* device time of the last render (events)      */
-    float composite_ms;         /* device time of the sort-last over pass        */
+    float composite_ms;         /* device time of the IVR_SORT_LAST_TILES over pass */
 } ivr_frame_stats;
 
 /* ---- context ---- */

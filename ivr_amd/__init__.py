@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libivr.so")
 
 IVR_F32, IVR_F16 = 0, 1
 MODE_COMPAT, MODE_CONFIG = 0, 1
-FUSED, SORT_LAST = 0, 1
+FUSED, SORT_LAST, SORT_LAST_TILES = 0, 1, 2
 GEN_BLOBS, GEN_NOISE_BLOBS = 0, 1
 
 # reference transfer function ids (transferfuncs.h:8-13)

@@ -13,14 +13,15 @@ ERT = 0.99
 
 CONFIGS = {
     1: dict(name="C1 64^3 fp32 blobs -> 256^2", dims=(64, 64, 64), dtype=ivr.IVR_F32, brick=64,
-            image=(256, 256), skip=False,
+            image=(256, 256), skip=False, composite=ivr.FUSED,
             gen=dict(kind=ivr.GEN_BLOBS, nblobs=8)),
-    2: dict(name="C2 512^3 fp32 Perlin+blobs (40% zero), 2x2x2 bricks of 256^3 -> 1024^2",
+    2: dict(name="C2 512^3 fp32 Perlin+blobs (40% zero), 2x2x2 bricks of 256^3, sort-last -> 1024^2",
             dims=(512, 512, 512), dtype=ivr.IVR_F32, brick=256, image=(1024, 1024), skip=False,
+            composite=ivr.SORT_LAST,
             gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.4)),
     3: dict(name="C3 1024^3 fp16 Perlin+blobs (70% zero), 4x4x4 bricks of 256^3 -> 1920x1080",
             dims=(1024, 1024, 1024), dtype=ivr.IVR_F16, brick=256, image=(1920, 1080), skip=True,
-            gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.7)),
+            composite=ivr.FUSED, gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.7)),
 }
 
 

@@ -402,7 +402,9 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
         return ivr_fail(IVR_ERR_ARG, "unknown mode %d", d->mode);
     if (d->mode == IVR_MODE_CONFIG && !(d->step_vox > 0.0f))
         return ivr_fail(IVR_ERR_ARG, "step_vox must be > 0");
-    if (d->mode == IVR_MODE_COMPAT && d->composite == IVR_SORT_LAST)
+    if (d->composite != IVR_FUSED && d->composite != IVR_SORT_LAST && d->composite != IVR_SORT_LAST_TILES)
+        return ivr_fail(IVR_ERR_ARG, "unknown composite %d", d->composite);
+    if (d->mode == IVR_MODE_COMPAT && d->composite != IVR_FUSED)
         return ivr_fail(IVR_ERR_UNSUPPORTED, "sort-last compositing is a config-mode path");
     if ((size_t)tf->n * sizeof(float4) + (size_t)v->nbricks * 8 > 65536)
         return ivr_fail(IVR_ERR_UNSUPPORTED, "LUT of %d entries and %d bricks exceed 64 KiB of LDS", tf->n,
@@ -418,12 +420,13 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
     int rc;
     const int skip = d->mode == IVR_MODE_CONFIG && d->skip;
     if (skip && (rc = ivr_launch_cells(ctx, v, a))) return rc;
-    if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST) {
+    if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST_TILES) {
         if ((rc = render_sort_last(ctx, v, d, cam, a, f))) return rc;
         IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
         return IVR_OK;
     }
-    if ((rc = ivr_launch_render(ctx, v, a, skip))) return rc;
+    if ((rc = ivr_launch_render(ctx, v, a, skip, d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST)))
+        return rc;
     IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
     return IVR_OK;
 }

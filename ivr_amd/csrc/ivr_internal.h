@@ -155,7 +155,7 @@ void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const
 void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h);
 
 /* launchers (render.hip) */
-int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);
+int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last);
 int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a);
 int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out);
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a);

@@ -30,7 +30,7 @@ namespace {
 
 using namespace ivr_dev;
 
-template <int MODE, typename VT, bool SKIP>
+template <int MODE, typename VT, bool SKIP, bool SORT_LAST>
 __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
 {
     extern __shared__ float4 smem[];
@@ -46,7 +46,8 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
         if (MODE == IVR_MODE_CONFIG) {
             float d[3];
             cfg_dir(a, i, j, d);
-            march_config<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+            if (SORT_LAST) march_sort_last<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+            else march_config<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
         } else if (i >= a.rect[0] && i < a.rect[1] && j >= a.rect[2] && j < a.rect[3]) {
             /* rows by multiply, columns by repeated addition (main.c:987-999) */
             const float fi = (float)(i - a.rect[0]);
@@ -198,11 +199,11 @@ size_t smem_bytes(const RenderArgs &a)
     return (size_t)a.lutsize * sizeof(float4) + (size_t)a.nbricks * sizeof(unsigned long long);
 }
 
-template <int MODE, typename VT, bool SKIP>
+template <int MODE, typename VT, bool SKIP, bool SORT_LAST = false>
 void launch_one(const RenderArgs &a, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_render<MODE, VT, SKIP>), dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a),
-                       s, a);
+    hipLaunchKernelGGL((k_render<MODE, VT, SKIP, SORT_LAST>), dim3(a.nblk_pad), dim3(IVR_BLOCK),
+                       smem_bytes(a), s, a);
 }
 
 } // namespace
@@ -220,7 +221,7 @@ void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h)
     a.wave_shift = ctx->wave_w == 4 ? 2 : ctx->wave_w == 16 ? 4 : 3;
 }
 
-int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip)
+int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last)
 {
     ivr_set_grid(a, ctx, a.W, a.H);
     const bool f16 = vol->dtype == IVR_F16;
@@ -228,6 +229,14 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
     if (a.mode == IVR_MODE_COMPAT) {
         if (f16) launch_one<IVR_MODE_COMPAT, h16, false>(a, s);
         else launch_one<IVR_MODE_COMPAT, float, false>(a, s);
+    } else if (sort_last) {
+        if (f16) {
+            if (skip) launch_one<IVR_MODE_CONFIG, h16, true, true>(a, s);
+            else launch_one<IVR_MODE_CONFIG, h16, false, true>(a, s);
+        } else {
+            if (skip) launch_one<IVR_MODE_CONFIG, float, true, true>(a, s);
+            else launch_one<IVR_MODE_CONFIG, float, false, true>(a, s);
+        }
     } else if (skip) {
         if (f16) launch_one<IVR_MODE_CONFIG, h16, true>(a, s);
         else launch_one<IVR_MODE_CONFIG, float, true>(a, s);

@@ -150,37 +150,54 @@ __device__ __forceinline__ void ftb_sample(const RenderArgs &a, const float4 *s_
     acc.a = acc.a + w;
 }
 
-/* front-to-back march of one ray over the global sample grid */
-template <typename VT, bool SKIP>
-__device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *s_lut,
-                                             const unsigned long long *s_boff, const float *o,
-                                             const float *d, Acc &acc, unsigned &cnt,
-                                             unsigned &fetched)
+/* One config-mode ray on the global sample grid t_k = tin + k dt, k < n. */
+struct Ray {
+    const float *o;
+    float d[3];
+    float tin, dt;
+    int n;
+    float go[3], gd[3], rgd[3]; /* voxel-space ray g(t) = go + gd t (macrocell exits) */
+};
+
+/* slab test and sample count of the ray through pixel direction d; false
+ * when it misses the volume */
+__device__ __forceinline__ bool ray_setup(const RenderArgs &a, const float *o, const float *d, Ray &r)
 {
     float tin, tout;
-    if (!slab(a.dconst, o, d, tin, tout)) return;
-    tin = fmaxf(tin, 0.0f);
-    const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    const float dt = (a.vox * a.step_vox) / dlen;
-    const float nf = 1.0f + (tout - tin) / dt + 0.5f;
-    /* guard for degenerate directions; never reached by a finite ray */
-    const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
-    float go[3], rgd[3], gd[3];
-    if (SKIP) {
+    if (!slab(a.dconst, o, d, tin, tout)) return false;
+    r.o = o;
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            go[r] = (o[r] - a.bmin[r]) * a.s[r];
-            gd[r] = d[r] * a.s[r];
-            rgd[r] = 1.0f / gd[r];
-        }
+    for (int q = 0; q < 3; q++) r.d[q] = d[q];
+    r.tin = fmaxf(tin, 0.0f);
+    const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    r.dt = (a.vox * a.step_vox) / dlen;
+    const float nf = 1.0f + (tout - r.tin) / r.dt + 0.5f;
+    /* guard for degenerate directions; never reached by a finite ray */
+    r.n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        r.go[q] = (o[q] - a.bmin[q]) * a.s[q];
+        r.gd[q] = d[q] * a.s[q];
+        r.rgd[q] = 1.0f / r.gd[q];
     }
-    /* One loop for skipping and sampling, so that within a wavefront the
-     * rays crossing empty cells and the rays sampling advance together. */
-    int k = 0;
+    return true;
+}
+
+/* Front-to-back march over samples [k0, k1) from acc, with early ray
+ * termination.  One loop for skipping and sampling, so that within a
+ * wavefront the rays crossing empty cells and the rays sampling advance
+ * together.  cnt counts composited and skipped samples, fetched the
+ * samples that were classified. */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 *s_lut,
+                                              const unsigned long long *s_boff, const Ray &r, int k0,
+                                              int k1, Acc &acc, unsigned &cnt, unsigned &fetched)
+{
+    int k = k0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     float v[8];
-    while (k < n) {
-        const Pos p = cfg_pos(a, o, d, tin, dt, k);
+    while (k < k1) {
+        const Pos p = cfg_pos(a, r.o, r.d, r.tin, r.dt, k);
         if (SKIP) {
             const int cell = cell_of(a, p);
             if (a.cellempty[cell]) {
@@ -189,18 +206,18 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
                 const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
                 float te = FLT_MAX;
 #pragma unroll
-                for (int r = 0; r < 3; r++) {
-                    if (gd[r] != 0.0f) {
-                        const float face = (float)((cc[r] + (gd[r] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
-                        te = fminf(te, (face - go[r]) * rgd[r]);
+                for (int q = 0; q < 3; q++) {
+                    if (r.gd[q] != 0.0f) {
+                        const float face = (float)((cc[q] + (r.gd[q] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
+                        te = fminf(te, (face - r.go[q]) * r.rgd[q]);
                     }
                 }
                 int kc = k + 1;
-                const float q = (te - tin) / dt;
-                if (q > (float)(k + 1)) {
-                    int cand = q < (float)n ? (int)q : n;
+                const float qk = (te - r.tin) / r.dt;
+                if (qk > (float)(k + 1)) {
+                    const int cand = qk < (float)k1 ? (int)qk : k1;
                     if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, o, d, tin, dt, cand - 1);
+                        const Pos pv = cfg_pos(a, r.o, r.d, r.tin, r.dt, cand - 1);
                         if (cell_of(a, pv) == cell) kc = cand;
                     }
                 }
@@ -220,6 +237,59 @@ __device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *
         fetched++;
         k++;
         if (acc.a >= a.ert) break;
+    }
+}
+
+/* front-to-back march of one ray over the global sample grid */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_config(const RenderArgs &a, const float4 *s_lut,
+                                             const unsigned long long *s_boff, const float *o,
+                                             const float *d, Acc &acc, unsigned &cnt,
+                                             unsigned &fetched)
+{
+    Ray r;
+    if (!ray_setup(a, o, d, r)) return;
+    march_segment<VT, SKIP>(a, s_lut, s_boff, r, 0, r.n, acc, cnt, fetched);
+}
+
+__device__ __forceinline__ int brick_of(const RenderArgs &a, const Pos &p)
+{
+    return (p.j >> a.bshift) + a.nb[0] * ((p.i >> a.bshift) + a.nb[1] * (p.k >> a.bshift));
+}
+
+/* Sort-last compositing of one ray without materialised tiles.  A brick's
+ * partial is the front-to-back composite (from zero, early termination
+ * inside the brick) of the samples whose floor index it owns; the partials
+ * are combined far to near with d = s + (1 - s.a) d.  Each axis index of
+ * the samples is monotone in k, so a brick owns one contiguous run of k and
+ * the runs come in visibility order: walking the runs from the far end and
+ * finding each run's start by bisection gives exactly the tiles and the
+ * order of the materialised scheme. */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float4 *s_lut,
+                                                const unsigned long long *s_boff, const float *o,
+                                                const float *d, Acc &out, unsigned &cnt,
+                                                unsigned &fetched)
+{
+    Ray r;
+    if (!ray_setup(a, o, d, r)) return;
+    int kend = r.n;
+    while (kend > 0) {
+        const int b = brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, kend - 1));
+        int lo = 0, hi = kend - 1; /* first k of the run that ends at kend - 1 */
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, mid)) == b) hi = mid;
+            else lo = mid + 1;
+        }
+        Acc s = {0.0f, 0.0f, 0.0f, 0.0f};
+        march_segment<VT, SKIP>(a, s_lut, s_boff, r, lo, kend, s, cnt, fetched);
+        /* compositetiles' "over" (main.c:1206-1215), component order as there */
+        out.b = s.b + (1.0f - s.a) * out.b;
+        out.g = s.g + (1.0f - s.a) * out.g;
+        out.r = s.r + (1.0f - s.a) * out.r;
+        out.a = s.a + (1.0f - s.a) * out.a;
+        kend = lo;
     }
 }
 

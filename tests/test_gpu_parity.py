@@ -34,6 +34,29 @@ def _render(ctx, vol, lut, desc, W, H):
     return img8, img32, st
 
 
+def _oracle_sort_last(host, st, lut, B, rows=None):
+    """oracle tiles of every brick (near to far by the sum of axis distances
+    to the eye's brick, ties by id) composited by compositetiles"""
+    import ctypes as C
+    n = host.shape[0]
+    v = oracle.view_of(st)
+    cam = np.zeros(23, np.float32)
+    rect = (C.c_int * 4)()
+    bmin, bmax = oracle.box(n, n, n)
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
+    eb = [int(np.floor((cam[r] - bmin[r]) * (n / (bmax[r] - bmin[r])) / B)) for r in range(3)]
+    nb = (n - 2) // B + 1
+    ids = sorted([(x, y, z) for z in range(nb) for y in range(nb) for x in range(nb)],
+                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb * (b[1] + nb * b[2])))
+    tiles, s_ref = [], 0
+    for b in ids:
+        t, s = oracle.render(host, v, lut, oracle.params(brick=B, brick_id=b), rows=rows)
+        tiles.append(t)
+        s_ref += s
+    return oracle.composite_tiles(np.stack(tiles)), s_ref
+
+
 def _report(tag, img8, ref32, img32=None):
     ref8 = oracle.quantize(ref32)
     d8 = np.abs(img8.astype(int) - ref8.astype(int))
@@ -142,9 +165,11 @@ def test_bricking_and_skipping_are_invisible(ctx):
     assert s_ref == base_st["samples"]
 
 
-def test_sort_last_matches_oracle_tiles(ctx):
-    """per-brick tiles composited in visibility order == the oracle's tile
-    renders composited by compositetiles, bit for bit"""
+@pytest.mark.parametrize("composite", ["SORT_LAST", "SORT_LAST_TILES"])
+def test_sort_last_matches_oracle_tiles(ctx, composite):
+    """per-brick partials composited in visibility order == the oracle's tile
+    renders composited by compositetiles, bit for bit, for the register
+    scheme and the materialised tiles; macrocell skipping invisible"""
     gp = configs.gen_params(2)
     n, B = 64, 16
     host = ivr.generate_host(gp, n, n, n)
@@ -152,25 +177,15 @@ def test_sort_last_matches_oracle_tiles(ctx):
     st = configs.state(2, 96, 80)
     vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=B)
     vol.upload(host)
-    img8, img32, stats = _render(ctx, vol, lut, ivr.config_desc(st, composite=ivr.SORT_LAST), 96, 80)
+    mode = getattr(ivr, composite)
+    img8, img32, stats = _render(ctx, vol, lut, ivr.config_desc(st, composite=mode), 96, 80)
+    if composite == "SORT_LAST":
+        s8, s32, sstats = _render(ctx, vol, lut, ivr.config_desc(st, composite=mode, skip=True), 96, 80)
+        assert np.array_equal(s32.view(np.uint32), img32.view(np.uint32))
+        assert sstats["samples"] == stats["samples"] and sstats["visited"] <= stats["visited"]
+        print("sort-last with skipping: %d of %d samples fetched" % (sstats["visited"], stats["visited"]))
     vol.close()
-    v = oracle.view_of(st)
-    import ctypes as C
-    cam = np.zeros(23, np.float32)
-    rect = (C.c_int * 4)()
-    bmin, bmax = oracle.box(n, n, n)
-    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
-    oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
-    eb = [int(np.floor((cam[r] - bmin[r]) * (n / (bmax[r] - bmin[r])) / B)) for r in range(3)]
-    nb = (n - 2) // B + 1
-    ids = sorted([(x, y, z) for z in range(nb) for y in range(nb) for x in range(nb)],
-                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb * (b[1] + nb * b[2])))
-    tiles, s_ref = [], 0
-    for b in ids:
-        t, s = oracle.render(host, v, lut, oracle.params(brick=B, brick_id=b))
-        tiles.append(t)
-        s_ref += s
-    ref32 = oracle.composite_tiles(np.stack(tiles))
+    ref32, s_ref = _oracle_sort_last(host, st, lut, B)
     m, rmse, same = _report("sort-last vs oracle tiles", img8, ref32, img32)
     assert same == 1.0 and stats["samples"] == s_ref
     # and within the ERT bound of the fused (global ERT) render
@@ -203,8 +218,17 @@ def test_c2_full_size_rows_match_oracle(ctx):
     sl = slice(*rows)
     m, rmse, same = _report("C2 rows %s vs oracle" % (rows,), a8[sl], ref32[sl], a32[sl])
     print("C2 max |gpu - fp64 oracle| = %.3e" % float(np.abs(a32[sl] - ref64[sl]).max()), flush=True)
-    assert m <= 1
+    assert same == 1.0
     assert (a8[..., 3] > 0).mean() > 0.2
+    # the C2 headline composite: sort-last over the 8 bricks
+    ds = ivr.config_desc(st, composite=ivr.SORT_LAST)
+    c8, c32, s3 = _render(ctx, vol, lut, ds, 1024, 1024)
+    rows = (480, 512)
+    sl = slice(*rows)
+    ref32, _ = _oracle_sort_last(host, st, lut, cfg["brick"], rows=rows)
+    m, rmse, same = _report("C2 sort-last rows %s vs oracle tiles" % (rows,), c8[sl], ref32[sl], c32[sl])
+    assert same == 1.0
+    print("C2 samples per frame: fused %d, sort-last %d" % (s1["samples"], s3["samples"]), flush=True)
     vol.close()
 
 
