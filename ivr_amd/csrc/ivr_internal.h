@@ -17,7 +17,8 @@
 #define IVR_MAX_LUT 4096
 #define IVR_MAX_BRICKS 4096
 #define IVR_MAX_STEPS (1 << 22) /* per-ray sample bound */
-#define IVR_BLOCK 256 /* 4 wavefronts = 2x2 tiles of 8x8 pixels */
+#define IVR_BLOCK 256 /* 4 wavefronts per 16x16 pixel block */
+#define IVR_SL_LDS 16384 /* LDS per workgroup for sort-last partials (4 runs) */
 
 int ivr_fail(int code, const char *fmt, ...);
 #define IVR_HIP(expr)                                                                          \

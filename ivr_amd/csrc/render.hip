@@ -30,12 +30,15 @@ namespace {
 
 using namespace ivr_dev;
 
-template <int MODE, typename VT, bool SKIP, bool SORT_LAST>
+/* SL: 0 fused, 1 sort-last with the partials in LDS, 2 sort-last walking
+ * the runs from the far end */
+template <int MODE, typename VT, bool SKIP, int SL>
 __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
 {
     extern __shared__ float4 smem[];
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
+    float4 *s_part = smem + a.lutsize + (a.nbricks + 1) / 2;
     int i, j;
     if (!block_pixel(a, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
@@ -46,7 +49,8 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
         if (MODE == IVR_MODE_CONFIG) {
             float d[3];
             cfg_dir(a, i, j, d);
-            if (SORT_LAST) march_sort_last<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+            if (SL == 1) march_sort_last<VT, SKIP>(a, s_lut, s_boff, s_part, a.o, d, acc, cnt, fetched);
+            else if (SL == 2) march_sort_last_rev<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
             else march_config<VT, SKIP>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
         } else if (i >= a.rect[0] && i < a.rect[1] && j >= a.rect[2] && j < a.rect[3]) {
             /* rows by multiply, columns by repeated addition (main.c:987-999) */
@@ -194,16 +198,17 @@ __global__ void k_cells(const float2 *range, unsigned char *empty, size_t ncells
     empty[c] = e;
 }
 
-size_t smem_bytes(const RenderArgs &a)
+size_t smem_bytes(const RenderArgs &a, int runs = 0)
 {
-    return (size_t)a.lutsize * sizeof(float4) + (size_t)a.nbricks * sizeof(unsigned long long);
+    return (size_t)a.lutsize * sizeof(float4) + (size_t)((a.nbricks + 1) / 2) * sizeof(float4) +
+           (size_t)runs * IVR_BLOCK * sizeof(float4);
 }
 
-template <int MODE, typename VT, bool SKIP, bool SORT_LAST = false>
-void launch_one(const RenderArgs &a, hipStream_t s)
+template <int MODE, typename VT, bool SKIP, int SL = 0>
+void launch_one(const RenderArgs &a, hipStream_t s, int runs = 0)
 {
-    hipLaunchKernelGGL((k_render<MODE, VT, SKIP, SORT_LAST>), dim3(a.nblk_pad), dim3(IVR_BLOCK),
-                       smem_bytes(a), s, a);
+    hipLaunchKernelGGL((k_render<MODE, VT, SKIP, SL>), dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a, runs),
+                       s, a);
 }
 
 } // namespace
@@ -230,12 +235,22 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
         if (f16) launch_one<IVR_MODE_COMPAT, h16, false>(a, s);
         else launch_one<IVR_MODE_COMPAT, float, false>(a, s);
     } else if (sort_last) {
-        if (f16) {
-            if (skip) launch_one<IVR_MODE_CONFIG, h16, true, true>(a, s);
-            else launch_one<IVR_MODE_CONFIG, h16, false, true>(a, s);
+        /* partials in LDS while they fit in IVR_SL_LDS bytes per workgroup */
+        const int runs = a.nb[0] + a.nb[1] + a.nb[2] - 2;
+        if ((size_t)runs * IVR_BLOCK * sizeof(float4) <= IVR_SL_LDS && smem_bytes(a, runs) <= 65536) {
+            if (f16) {
+                if (skip) launch_one<IVR_MODE_CONFIG, h16, true, 1>(a, s, runs);
+                else launch_one<IVR_MODE_CONFIG, h16, false, 1>(a, s, runs);
+            } else {
+                if (skip) launch_one<IVR_MODE_CONFIG, float, true, 1>(a, s, runs);
+                else launch_one<IVR_MODE_CONFIG, float, false, 1>(a, s, runs);
+            }
+        } else if (f16) {
+            if (skip) launch_one<IVR_MODE_CONFIG, h16, true, 2>(a, s);
+            else launch_one<IVR_MODE_CONFIG, h16, false, 2>(a, s);
         } else {
-            if (skip) launch_one<IVR_MODE_CONFIG, float, true, true>(a, s);
-            else launch_one<IVR_MODE_CONFIG, float, false, true>(a, s);
+            if (skip) launch_one<IVR_MODE_CONFIG, float, true, 2>(a, s);
+            else launch_one<IVR_MODE_CONFIG, float, false, 2>(a, s);
         }
     } else if (skip) {
         if (f16) launch_one<IVR_MODE_CONFIG, h16, true>(a, s);

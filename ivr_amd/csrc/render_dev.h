@@ -257,19 +257,151 @@ __device__ __forceinline__ int brick_of(const RenderArgs &a, const Pos &p)
     return (p.j >> a.bshift) + a.nb[0] * ((p.i >> a.bshift) + a.nb[1] * (p.k >> a.bshift));
 }
 
+/* First sample index after k (k < r.n, owned by brick b) whose floor index
+ * lies outside brick b, or r.n.  Each axis index of the samples is monotone
+ * in k, so a brick owns one contiguous run of k: the run end is estimated
+ * from the ray's crossing of the brick faces and then settled exactly by
+ * galloping and bisection on brick_of. */
+__device__ __forceinline__ int run_end(const RenderArgs &a, const Ray &r, int k, int b)
+{
+    const int bc[3] = {b % a.nb[0], (b / a.nb[0]) % a.nb[1], b / (a.nb[0] * a.nb[1])};
+    float te = FLT_MAX;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        /* the first and last bricks own the clamped samples beyond the faces */
+        const bool up = r.gd[q] > 0.0f;
+        if (r.gd[q] == 0.0f || (up && bc[q] == a.nb[q] - 1) || (!up && bc[q] == 0)) continue;
+        const float face = (float)((bc[q] + (up ? 1 : 0)) << a.bshift);
+        te = fminf(te, (face - r.go[q]) * r.rgd[q]);
+    }
+    int e = r.n;
+    const float qk = ceilf((te - r.tin) / r.dt);
+    if (qk < (float)r.n) e = max(k + 1, (int)qk);
+    auto owned = [&](int kk) { return brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, kk)) == b; };
+    int lo, hi; /* owned(lo), and hi == r.n or !owned(hi) */
+    if (e < r.n && owned(e)) {
+        lo = e; /* estimate short: gallop forward */
+        for (int step = 1;; step <<= 1) {
+            hi = min(lo + step, r.n);
+            if (hi == r.n || !owned(hi)) break;
+            lo = hi;
+        }
+    } else {
+        hi = e; /* estimate at or past the end: gallop backward */
+        lo = hi - 1;
+        for (int step = 1; lo > k && !owned(lo); step <<= 1) {
+            hi = lo;
+            lo = max(hi - step, k);
+        }
+    }
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (owned(mid)) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
 /* Sort-last compositing of one ray without materialised tiles.  A brick's
  * partial is the front-to-back composite (from zero, early termination
  * inside the brick) of the samples whose floor index it owns; the partials
- * are combined far to near with d = s + (1 - s.a) d.  Each axis index of
- * the samples is monotone in k, so a brick owns one contiguous run of k and
- * the runs come in visibility order: walking the runs from the far end and
- * finding each run's start by bisection gives exactly the tiles and the
- * order of the materialised scheme. */
+ * are combined far to near with compositetiles' d = s + (1 - s.a) d
+ * (main.c:1206-1215, component order as there).  Each axis index of the
+ * samples is monotone in k, so a brick owns one contiguous run of k and the
+ * runs come in visibility order: the per-ray run order is the materialised
+ * scheme's tile order restricted to the bricks the ray crosses, and bricks
+ * it misses contribute exact zeros there. */
+
+__device__ __forceinline__ void over(Acc &d, const Acc &s)
+{
+    d.b = s.b + (1.0f - s.a) * d.b;
+    d.g = s.g + (1.0f - s.a) * d.g;
+    d.r = s.r + (1.0f - s.a) * d.r;
+    d.a = s.a + (1.0f - s.a) * d.a;
+}
+
+/* Front-to-back walk in one pass: the march goes through the runs in
+ * order, a saturated run is left at its end (run_end), and the partials
+ * wait in LDS (s_part[run * blockDim.x + thread], at most nb0+nb1+nb2-2
+ * runs) until the far-to-near combination. */
 template <typename VT, bool SKIP>
 __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float4 *s_lut,
-                                                const unsigned long long *s_boff, const float *o,
-                                                const float *d, Acc &out, unsigned &cnt,
+                                                const unsigned long long *s_boff, float4 *s_part,
+                                                const float *o, const float *d, Acc &out, unsigned &cnt,
                                                 unsigned &fetched)
+{
+    Ray r;
+    if (!ray_setup(a, o, d, r)) return;
+    const int stride = blockDim.x, tid = threadIdx.x;
+    int nrun = 0, run_b = -1;
+    Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    int k = 0;
+    int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
+    float v[8];
+    while (k < r.n) {
+        const Pos p = cfg_pos(a, r.o, r.d, r.tin, r.dt, k);
+        const int b = brick_of(a, p);
+        if (b != run_b) {
+            if (run_b >= 0) s_part[(nrun++) * stride + tid] = make_float4(acc.r, acc.g, acc.b, acc.a);
+            acc.r = acc.g = acc.b = acc.a = 0.0f;
+            run_b = b;
+        }
+        if (SKIP) {
+            const int cell = cell_of(a, p);
+            if (a.cellempty[cell]) {
+                /* transparent cell (never wider than a brick): jump to the
+                 * first sample past its exit, verified exactly on the grid */
+                const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
+                float te = FLT_MAX;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    if (r.gd[q] != 0.0f) {
+                        const float face = (float)((cc[q] + (r.gd[q] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
+                        te = fminf(te, (face - r.go[q]) * r.rgd[q]);
+                    }
+                }
+                int kc = k + 1;
+                const float qk = (te - r.tin) / r.dt;
+                if (qk > (float)(k + 1)) {
+                    const int cand = qk < (float)r.n ? (int)qk : r.n;
+                    if (cand > k + 1) {
+                        const Pos pv = cfg_pos(a, r.o, r.d, r.tin, r.dt, cand - 1);
+                        if (cell_of(a, pv) == cell) kc = cand;
+                    }
+                }
+                cnt += (unsigned)(kc - k);
+                k = kc;
+                continue;
+            }
+        }
+        if (p.j != cj || p.i != ci || p.k != ck) {
+            fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+            cj = p.j;
+            ci = p.i;
+            ck = p.k;
+        }
+        ftb_sample(a, s_lut, v, p, acc);
+        cnt++;
+        fetched++;
+        /* early termination inside the brick: continue at the next run */
+        k = acc.a >= a.ert ? run_end(a, r, k, b) : k + 1;
+    }
+    if (run_b >= 0) s_part[(nrun++) * stride + tid] = make_float4(acc.r, acc.g, acc.b, acc.a);
+    for (int q = nrun - 1; q >= 0; q--) {
+        const float4 t = s_part[q * stride + tid];
+        const Acc s = {t.x, t.y, t.z, t.w};
+        over(out, s);
+    }
+}
+
+/* The same images for brick grids whose run count would not fit in LDS:
+ * the runs are walked from the far end, each run's start found by
+ * bisection, so only the running combination is kept. */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_sort_last_rev(const RenderArgs &a, const float4 *s_lut,
+                                                    const unsigned long long *s_boff, const float *o,
+                                                    const float *d, Acc &out, unsigned &cnt,
+                                                    unsigned &fetched)
 {
     Ray r;
     if (!ray_setup(a, o, d, r)) return;
@@ -284,11 +416,7 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
         }
         Acc s = {0.0f, 0.0f, 0.0f, 0.0f};
         march_segment<VT, SKIP>(a, s_lut, s_boff, r, lo, kend, s, cnt, fetched);
-        /* compositetiles' "over" (main.c:1206-1215), component order as there */
-        out.b = s.b + (1.0f - s.a) * out.b;
-        out.g = s.g + (1.0f - s.a) * out.g;
-        out.r = s.r + (1.0f - s.a) * out.r;
-        out.a = s.a + (1.0f - s.a) * out.a;
+        over(out, s);
         kend = lo;
     }
 }

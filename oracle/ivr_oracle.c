@@ -822,6 +822,16 @@ long long ivro_render_f64(const float *vol, int nx, int ny, int nz, const ivro_v
                     double gx = (o[0] + d[0] * t - bmin[0]) * sx;
                     double gy = (o[1] + d[1] * t - bmin[1]) * sy;
                     double gz = (o[2] + d[2] * t - bmin[2]) * sz;
+                    if (p->brick > 0) { /* tile mode: the brick's owned samples only */
+                        int j = (int)fmin(fmax(gx, 0.0), nx - 1.0), i = (int)fmin(fmax(gy, 0.0), ny - 1.0),
+                            kk = (int)fmin(fmax(gz, 0.0), nz - 1.0);
+                        j = j > nx - 2 ? nx - 2 : j;
+                        i = i > ny - 2 ? ny - 2 : i;
+                        kk = kk > nz - 2 ? nz - 2 : kk;
+                        if (j / p->brick != p->brick_id[0] || i / p->brick != p->brick_id[1] ||
+                            kk / p->brick != p->brick_id[2])
+                            continue;
+                    }
                     double s = tri_fetch64(vol, nx, ny, nz, gx, gy, gz);
                     int idx = (int)(s * p->lutscale) - p->lutoffset;
                     idx = idx < 0 ? 0 : (idx > lutsize - 1 ? lutsize - 1 : idx);

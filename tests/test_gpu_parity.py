@@ -165,13 +165,14 @@ def test_bricking_and_skipping_are_invisible(ctx):
     assert s_ref == base_st["samples"]
 
 
-@pytest.mark.parametrize("composite", ["SORT_LAST", "SORT_LAST_TILES"])
-def test_sort_last_matches_oracle_tiles(ctx, composite):
+@pytest.mark.parametrize("composite,B", [("SORT_LAST", 32), ("SORT_LAST", 16), ("SORT_LAST_TILES", 16)])
+def test_sort_last_matches_oracle_tiles(ctx, composite, B):
     """per-brick partials composited in visibility order == the oracle's tile
-    renders composited by compositetiles, bit for bit, for the register
-    scheme and the materialised tiles; macrocell skipping invisible"""
+    renders composited by compositetiles, bit for bit: one-launch sort-last
+    with 2^3 bricks (partials in LDS) and 4^3 bricks (runs walked from the
+    far end), and the materialised tiles; macrocell skipping invisible"""
     gp = configs.gen_params(2)
-    n, B = 64, 16
+    n = 64
     host = ivr.generate_host(gp, n, n, n)
     lut = configs.tf_lut(2)
     st = configs.state(2, 96, 80)

@@ -238,3 +238,23 @@ def test_fp64_oracle_close():
     err = np.abs(i32 - i64).max()
     assert err < 2.0 / 255.0, err
     assert abs(n32 - n64) <= 0.001 * n64
+
+
+def test_fp64_oracle_tiles_close():
+    """the FP64 oracle's tile mode keeps the brick's own samples: per-brick
+    tiles agree with the fp32 tiles and their samples partition the grid"""
+    n, B = 48, 16
+    gp = configs.gen_params(1)
+    vol = ivr.generate_host(gp, n, n, n)
+    lut = configs.tf_lut(1)
+    st = ivr.state_from_angles(2.2, -0.35, 40, 40)
+    v = oracle.view_of(st)
+    _, s_full = oracle.render(vol, v, lut, oracle.params(ert=2.0), f64=True)
+    nb = (n - 2) // B + 1
+    s_tiles = 0
+    for b in [(x, y, z) for z in range(nb) for y in range(nb) for x in range(nb)]:
+        t32, _ = oracle.render(vol, v, lut, oracle.params(ert=2.0, brick=B, brick_id=b))
+        t64, s = oracle.render(vol, v, lut, oracle.params(ert=2.0, brick=B, brick_id=b), f64=True)
+        s_tiles += s
+        assert np.abs(t64 - t32).max() < 1e-3, b
+    assert abs(s_tiles - s_full) <= 0.001 * s_full
