@@ -23,6 +23,10 @@ reference_cpu: the reference ray.c renderer compiled from its sources
 (oracle/_ref), single core, against this library's compat mode on the GPU
 over the same volume, camera and LUT (bit-identical images).
 
+--config 4 (learned surrogate of the C2 volume) and --config 5 (steering
+service) print their own JSON lines; they are measurements of those parts,
+not the headline.
+
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
 """
 import argparse
@@ -53,6 +57,8 @@ def main():
                     help="brick compositing (default per config)")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--train-seconds", type=float, default=60.0, help="--config 4: training time budget")
+    ap.add_argument("--requests", type=int, default=400, help="--config 5: requests per arrival rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (for scaling PMC byte counters)")
     args = ap.parse_args()
@@ -69,6 +75,12 @@ def main():
     import ivr_amd as ivr
     from ivr_amd import configs
 
+    if args.config == 4:
+        bench_surrogate(args)
+        return
+    if args.config == 5:
+        bench_serving(args)
+        return
     c = args.config
     cfg = configs.CONFIGS[c]
     W, H = cfg["image"]
@@ -173,6 +185,135 @@ def main():
     fr.close()
     tf.close()
     vol.close()
+    ctx.close()
+
+
+def _frame_ms(ctx, fn, steps):
+    fn()
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(steps):
+        fn()
+    ctx.record(1)
+    ctx.sync()
+    return ctx.elapsed(0, 1) / steps
+
+
+def bench_surrogate(args):
+    """C4: per-brick hash grids + BF16 MFMA MLP trained on the C2 volume
+    with 2^24 samples per Adam step for a time budget, then 1920x1080 frames
+    from the surrogate against the plain render: PSNR/SSIM, frames/s, and
+    the per-brick fallback below 40 dB."""
+    import numpy as np
+    import ivr_amd as ivr
+    from ivr_amd import configs
+    cfg = configs.CONFIGS[2]
+    ctx = ivr.Context(0)
+    vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
+    vol.generate(configs.gen_params(2))
+    W, H = 1920, 1080
+    lut = configs.tf_lut(2)
+    tf = ctx.tf(lut)
+    desc = ivr.config_desc(ivr.orbit_state(3004, W, H))  # camera seed of C4
+    fr = ctx.frame(W, H)
+    plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), args.steps)
+    plain = fr.download()
+    sg = ctx.surrogate(vol)
+    batch = 1 << 24
+    losses, steps, t_train = [], 0, 0.0
+    chunk = 5
+    while t_train < args.train_seconds:
+        ctx.sync()
+        t = time.perf_counter()
+        losses.append(sg.train(vol, chunk, batch))
+        ctx.sync()
+        t_train += time.perf_counter() - t
+        steps += chunk
+        log("surrogate: %d steps, %.1fs, loss %.3e" % (steps, t_train, losses[-1]))
+    step_ms = t_train * 1e3 / steps
+    sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
+    sg_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
+    img = fr.download()
+    p_sg, s_sg = ivr.psnr(img, plain), ivr.ssim(img, plain)
+    bpsnr, nfall = sg.evaluate(vol, tf, desc, threshold_db=40.0)
+    fb_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
+    img_fb = fr.download()
+    p_fb, s_fb = ivr.psnr(img_fb, plain), ivr.ssim(img_fb, plain)
+    log("surrogate: train %.1f ms/step (%d steps), render %.3f ms (plain %.3f); PSNR %.2f dB SSIM %.4f; "
+        "%d/%d bricks below 40 dB -> with fallback PSNR %.2f dB, %.3f ms"
+        % (step_ms, steps, sg_ms, plain_ms, p_sg, s_sg, nfall, sg.nbricks, p_fb, fb_ms))
+    flops_step = batch * 3 * 2 * (32 * 64 + 64 * 64 + 64)
+    out = {
+        "metric": "surrogate frames/s at 1920x1080 with per-brick fallback below 40 dB",
+        "value": round(1000.0 / fb_ms, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 MLP on MFMA, fp32 tables and accumulation",
+        "data": "synthetic C2 volume (512^3 fp32), surrogate trained on the GPU",
+        "config": {
+            "workload": "C4 surrogate of C2: 8 bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1",
+            "image": "%dx%d" % (W, H),
+            "train_steps": steps, "train_samples_per_step": batch, "train_ms_per_step": round(step_ms, 2),
+            "train_mlp_tflops": round(flops_step / (step_ms * 1e-3) / 1e12, 2),
+            "loss_first": losses[0], "loss_last": losses[-1],
+            "surrogate_ms": round(sg_ms, 3), "surrogate_fps": round(1000.0 / sg_ms, 2),
+            "psnr_db": round(p_sg, 2), "ssim": round(s_sg, 4),
+            "brick_psnr_db": [round(float(x), 2) for x in bpsnr],
+            "fallback_bricks": int(nfall),
+            "fallback_ms": round(fb_ms, 3), "fallback_psnr_db": round(p_fb, 2), "fallback_ssim": round(s_fb, 4),
+            "plain_ms": round(plain_ms, 3), "plain_fps": round(1000.0 / plain_ms, 2),
+        },
+    }
+    print(json.dumps(out), flush=True)
+    sg.close()
+    fr.close()
+    tf.close()
+    vol.close()
+    ctx.close()
+
+
+def bench_serving(args):
+    """C5: three resident volumes (256^3 fp32, C2 512^3 fp32, C3 1024^3 fp16)
+    behind the continuous batcher; Poisson arrivals at 50..400 req/s and an
+    all-at-once burst for capacity; each request a seeded orbit camera, a
+    jittered TF and a resolution from {256^2, 512^2, 1024^2, 1920x1080}."""
+    import ivr_amd as ivr
+    from ivr_amd import configs, serving
+    ctx = ivr.Context(0)
+    vols = []
+    for (n, dt, c, brick) in ((256, ivr.IVR_F32, 2, 256), (512, ivr.IVR_F32, 2, 256), (1024, ivr.IVR_F16, 3, 256)):
+        v = ctx.volume(n, n, n, dt, brick=brick)
+        v.generate(configs.gen_params(c))
+        vols.append(v)
+    bt = ctx.batcher(64 << 30, 16, 1920, 1080)
+    for v in vols:
+        bt.add_volume(v)
+    rows = []
+    for k, rate in enumerate((50.0, 100.0, 200.0, 400.0, 1e9)):
+        reqs = serving.make_requests(args.requests, len(vols), seed=4100 + k, tf_seed=2005)
+        res = serving.run_poisson(bt, reqs, rate, seed=4000 + k)
+        row = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in res.items() if kk != "records"}
+        row["rate"] = "burst" if rate >= 1e9 else rate
+        log("serving: %s" % row)
+        rows.append(row)
+    st = bt.stats()
+    out = {
+        "metric": "steering service p99 latency (arrival -> RGBA8 on host) and frames/s",
+        "value": rows[-2]["p99_ms"],
+        "unit": "ms (p99 at 400 req/s)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "data": "synthetic volumes generated on the GPU; seeded requests",
+        "config": {"workload": "C5: 256^3 fp32, 512^3 fp32, 1024^3 fp16 resident; 16 slots; 64 GiB budget",
+                   "requests_per_rate": args.requests, "rates": rows,
+                   "capacity_frames_per_s": rows[-1]["frames_per_s"],
+                   "batcher": {k: v for k, v in st.items()} if isinstance(st, dict) else None},
+    }
+    print(json.dumps(out), flush=True)
+    bt.close()
+    for v in vols:
+        v.close()
     ctx.close()
 
 
