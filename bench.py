@@ -232,7 +232,11 @@ def bench_surrogate(args):
         log("surrogate: %d steps, %.1fs, loss %.3e" % (steps, t_train, losses[-1]))
     step_ms = t_train * 1e3 / steps
     sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
+    ns_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
+    p_ns = ivr.psnr(fr.download(), plain)
+    desc = ivr.config_desc(desc.state, skip=True)  # the network sees only samples of non-transparent cells
     sg_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
+    sg_stats = fr.stats()
     img = fr.download()
     p_sg, s_sg = ivr.psnr(img, plain), ivr.ssim(img, plain)
     bpsnr, nfall = sg.evaluate(vol, tf, desc, threshold_db=40.0)
@@ -258,6 +262,9 @@ def bench_surrogate(args):
             "train_mlp_tflops": round(flops_step / (step_ms * 1e-3) / 1e12, 2),
             "loss_first": losses[0], "loss_last": losses[-1],
             "surrogate_ms": round(sg_ms, 3), "surrogate_fps": round(1000.0 / sg_ms, 2),
+            "surrogate_no_skip_ms": round(ns_ms, 3), "surrogate_no_skip_psnr_db": round(p_ns, 2),
+            "network_samples_per_frame": int(sg_stats["visited"]),
+            "network_gsamples_s": round(sg_stats["visited"] / sg_ms / 1e6, 3),
             "psnr_db": round(p_sg, 2), "ssim": round(s_sg, 4),
             "brick_psnr_db": [round(float(x), 2) for x in bpsnr],
             "fallback_bricks": int(nfall),

@@ -42,6 +42,13 @@ struct ivr_surrogate {
     float *d_loss = nullptr;
     unsigned char *d_fallback = nullptr;
     int step = 0;
+    /* value range of every 16^3 macrocell of the trained volume (2 floats
+     * per 4096 voxels): the surrogate skips cells that the frame's transfer
+     * function makes transparent, as the plain renderer does */
+    int nc[3] = {0, 0, 0};
+    size_t ncells = 0;
+    float2 *d_cellrange = nullptr;
+    unsigned char *d_cellempty = nullptr;
 };
 
 #endif

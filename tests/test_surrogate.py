@@ -164,9 +164,18 @@ def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
     print("losses", losses, "PSNR untrained %.2f dB trained %.2f dB SSIM %.4f" % (p0, p1, ivr.ssim(trained, plain)))
     assert losses[-1] < losses[0]
     assert p1 > p0 + 10.0
-    # all bricks forced to fall back: identical to the plain render
+    # macrocell skipping: the network only sees samples that can contribute
+    ds = ivr.config_desc(st, skip=True)
+    sg.render(vol, tf, ds, fr)
+    skipped = fr.download()
+    p2 = ivr.psnr(skipped, plain)
+    print("PSNR with macrocell skipping %.2f dB" % p2)
+    assert p2 > p1 - 1.0
+    # all bricks forced to fall back: identical to the plain render, with and without skipping
     sg.set_fallback(np.ones(sg.nbricks, np.uint8))
     sg.render(vol, tf, d, fr)
+    assert np.array_equal(fr.download(), plain)
+    sg.render(vol, tf, ds, fr)
     assert np.array_equal(fr.download(), plain)
     # per-brick evaluation at an unreachable threshold flags every brick
     psnr, nf = sg.evaluate(vol, tf, d, threshold_db=200.0)
