@@ -98,6 +98,8 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
             const float dt = (a.vox * a.step_vox) / dlen;
             const float nf = 1.0f + (tout - tin) / dt + 0.5f;
             const int n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
+            float gor[3], gdr[3];
+            ray_voxel(a, a.o, d, gor, gdr);
             /* k range of the brick's ownership box (open towards the volume
              * faces, where clamped samples belong to the boundary brick);
              * grazing rays scan the whole grid */
@@ -105,8 +107,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
             float lo = tin, hi = tin + (float)n * dt;
             bool graze = false;
             for (int r = 0; r < 3; r++) {
-                const float gd = d[r] * a.s[r];
-                const float go = (a.o[r] - a.bmin[r]) * a.s[r];
+                const float gd = gdr[r], go = gor[r];
                 if (fabsf(gd) * 20.0f < dlen * a.s[r]) { graze = true; continue; }
                 const float blo = a.tb[r] == 0 ? -FLT_MAX : (float)(a.tb[r] << a.bshift);
                 const float bhi = a.tb[r] == a.nb[r] - 1 ? FLT_MAX : (float)((a.tb[r] + 1) << a.bshift);
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
                 }
             }
             for (int k = k0; k < k1; k++) {
-                const Pos p = cfg_pos(a, a.o, d, tin, dt, k);
+                const Pos p = cfg_pos(a, gor, gdr, tin, dt, k);
                 if ((p.j >> a.bshift) != a.tb[0] || (p.i >> a.bshift) != a.tb[1] ||
                     (p.k >> a.bshift) != a.tb[2])
                     continue;

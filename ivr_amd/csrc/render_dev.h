@@ -97,16 +97,16 @@ struct Pos {
     float fx, fy, fz;
 };
 
-__device__ __forceinline__ Pos cfg_pos(const RenderArgs &a, const float *o, const float *d, float tin,
+/* sample k of a ray in voxel coordinates: g = gd t + go with
+ * gd = dir * N/ext, go = (eye - min) * N/ext precomputed per ray */
+__device__ __forceinline__ Pos cfg_pos(const RenderArgs &a, const float *go, const float *gd, float tin,
                                        float dt, int kk)
 {
     const float t = fmaf((float)kk, dt, tin);
-    float gx = (fmaf(d[0], t, o[0]) - a.bmin[0]) * a.s[0];
-    float gy = (fmaf(d[1], t, o[1]) - a.bmin[1]) * a.s[1];
-    float gz = (fmaf(d[2], t, o[2]) - a.bmin[2]) * a.s[2];
-    gx = fminf(fmaxf(gx, 0.0f), a.nm1f[0]);
-    gy = fminf(fmaxf(gy, 0.0f), a.nm1f[1]);
-    gz = fminf(fmaxf(gz, 0.0f), a.nm1f[2]);
+    /* clamp to [0, N-1]: v_med3 is min(max()) for the finite values here */
+    const float gx = __builtin_amdgcn_fmed3f(fmaf(gd[0], t, go[0]), 0.0f, a.nm1f[0]);
+    const float gy = __builtin_amdgcn_fmed3f(fmaf(gd[1], t, go[1]), 0.0f, a.nm1f[1]);
+    const float gz = __builtin_amdgcn_fmed3f(fmaf(gd[2], t, go[2]), 0.0f, a.nm1f[2]);
     Pos p;
     p.j = min((int)gx, a.n[0] - 2);
     p.i = min((int)gy, a.n[1] - 2);
@@ -115,6 +115,17 @@ __device__ __forceinline__ Pos cfg_pos(const RenderArgs &a, const float *o, cons
     p.fy = gy - (float)p.i;
     p.fz = gz - (float)p.k;
     return p;
+}
+
+/* per-ray voxel-space origin and direction of cfg_pos */
+__device__ __forceinline__ void ray_voxel(const RenderArgs &a, const float *o, const float *d, float *go,
+                                          float *gd)
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        go[r] = (o[r] - a.bmin[r]) * a.s[r];
+        gd[r] = d[r] * a.s[r];
+    }
 }
 
 __device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
@@ -152,11 +163,9 @@ __device__ __forceinline__ void ftb_sample(const RenderArgs &a, const float4 *s_
 
 /* One config-mode ray on the global sample grid t_k = tin + k dt, k < n. */
 struct Ray {
-    const float *o;
-    float d[3];
     float tin, dt;
     int n;
-    float go[3], gd[3], rgd[3]; /* voxel-space ray g(t) = go + gd t (macrocell exits) */
+    float go[3], gd[3], rgd[3]; /* voxel-space ray g(t) = go + gd t */
 };
 
 /* slab test and sample count of the ray through pixel direction d; false
@@ -165,21 +174,15 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs &a, const float *o, c
 {
     float tin, tout;
     if (!slab(a.dconst, o, d, tin, tout)) return false;
-    r.o = o;
-#pragma unroll
-    for (int q = 0; q < 3; q++) r.d[q] = d[q];
     r.tin = fmaxf(tin, 0.0f);
     const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     r.dt = (a.vox * a.step_vox) / dlen;
     const float nf = 1.0f + (tout - r.tin) / r.dt + 0.5f;
     /* guard for degenerate directions; never reached by a finite ray */
     r.n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
+    ray_voxel(a, o, d, r.go, r.gd);
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        r.go[q] = (o[q] - a.bmin[q]) * a.s[q];
-        r.gd[q] = d[q] * a.s[q];
-        r.rgd[q] = 1.0f / r.gd[q];
-    }
+    for (int q = 0; q < 3; q++) r.rgd[q] = 1.0f / r.gd[q];
     return true;
 }
 
@@ -197,7 +200,7 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     float v[8];
     while (k < k1) {
-        const Pos p = cfg_pos(a, r.o, r.d, r.tin, r.dt, k);
+        const Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
         if (SKIP) {
             const int cell = cell_of(a, p);
             if (a.cellempty[cell]) {
@@ -217,7 +220,7 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
                 if (qk > (float)(k + 1)) {
                     const int cand = qk < (float)k1 ? (int)qk : k1;
                     if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, r.o, r.d, r.tin, r.dt, cand - 1);
+                        const Pos pv = cfg_pos(a, r.go, r.gd, r.tin, r.dt, cand - 1);
                         if (cell_of(a, pv) == cell) kc = cand;
                     }
                 }
@@ -277,7 +280,7 @@ __device__ __forceinline__ int run_end(const RenderArgs &a, const Ray &r, int k,
     int e = r.n;
     const float qk = ceilf((te - r.tin) / r.dt);
     if (qk < (float)r.n) e = max(k + 1, (int)qk);
-    auto owned = [&](int kk) { return brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, kk)) == b; };
+    auto owned = [&](int kk) { return brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, kk)) == b; };
     int lo, hi; /* owned(lo), and hi == r.n or !owned(hi) */
     if (e < r.n && owned(e)) {
         lo = e; /* estimate short: gallop forward */
@@ -321,9 +324,9 @@ __device__ __forceinline__ void over(Acc &d, const Acc &s)
 }
 
 /* Front-to-back walk in one pass: the march goes through the runs in
- * order, a saturated run is left at its end (run_end), and the partials
- * wait in LDS (s_part[run * blockDim.x + thread], at most nb0+nb1+nb2-2
- * runs) until the far-to-near combination. */
+ * order, each run's end found once by run_end (a saturated run is left at
+ * its end), and the partials wait in LDS (s_part[run * blockDim.x +
+ * thread], at most nb0+nb1+nb2-2 runs) until the far-to-near combination. */
 template <typename VT, bool SKIP>
 __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float4 *s_lut,
                                                 const unsigned long long *s_boff, float4 *s_part,
@@ -333,18 +336,18 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
     Ray r;
     if (!ray_setup(a, o, d, r)) return;
     const int stride = blockDim.x, tid = threadIdx.x;
-    int nrun = 0, run_b = -1;
+    int nrun = 0, run_b = -1, run_e = 0; /* current run: brick, first index past it */
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
     int k = 0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     float v[8];
     while (k < r.n) {
-        const Pos p = cfg_pos(a, r.o, r.d, r.tin, r.dt, k);
-        const int b = brick_of(a, p);
-        if (b != run_b) {
+        const Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
+        if (k >= run_e) { /* first sample of the next run */
             if (run_b >= 0) s_part[(nrun++) * stride + tid] = make_float4(acc.r, acc.g, acc.b, acc.a);
             acc.r = acc.g = acc.b = acc.a = 0.0f;
-            run_b = b;
+            run_b = brick_of(a, p);
+            run_e = run_end(a, r, k, run_b);
         }
         if (SKIP) {
             const int cell = cell_of(a, p);
@@ -365,7 +368,7 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
                 if (qk > (float)(k + 1)) {
                     const int cand = qk < (float)r.n ? (int)qk : r.n;
                     if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, r.o, r.d, r.tin, r.dt, cand - 1);
+                        const Pos pv = cfg_pos(a, r.go, r.gd, r.tin, r.dt, cand - 1);
                         if (cell_of(a, pv) == cell) kc = cand;
                     }
                 }
@@ -384,7 +387,7 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
         cnt++;
         fetched++;
         /* early termination inside the brick: continue at the next run */
-        k = acc.a >= a.ert ? run_end(a, r, k, b) : k + 1;
+        k = acc.a >= a.ert ? run_e : k + 1;
     }
     if (run_b >= 0) s_part[(nrun++) * stride + tid] = make_float4(acc.r, acc.g, acc.b, acc.a);
     for (int q = nrun - 1; q >= 0; q--) {
@@ -407,11 +410,11 @@ __device__ __forceinline__ void march_sort_last_rev(const RenderArgs &a, const f
     if (!ray_setup(a, o, d, r)) return;
     int kend = r.n;
     while (kend > 0) {
-        const int b = brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, kend - 1));
+        const int b = brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, kend - 1));
         int lo = 0, hi = kend - 1; /* first k of the run that ends at kend - 1 */
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (brick_of(a, cfg_pos(a, r.o, r.d, r.tin, r.dt, mid)) == b) hi = mid;
+            if (brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, mid)) == b) hi = mid;
             else lo = mid + 1;
         }
         Acc s = {0.0f, 0.0f, 0.0f, 0.0f};

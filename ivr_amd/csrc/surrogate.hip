@@ -832,13 +832,10 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
     int k = 0;
     const float invB = 1.0f / (float)(1 << a.bshift);
     float go[3], gd[3], rgd[3];
+    ray_voxel(a, a.o, d, go, gd);
     if (SKIP) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            go[q] = (a.o[q] - a.bmin[q]) * a.s[q];
-            gd[q] = d[q] * a.s[q];
-            rgd[q] = 1.0f / gd[q];
-        }
+        for (int q = 0; q < 3; q++) rgd[q] = 1.0f / gd[q];
     }
     while (__ballot(live) != 0ull) {
         float x[3] = {0.5f, 0.5f, 0.5f};
@@ -846,7 +843,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
         Pos p;
         if (SKIP && live) {
             while (k < n) {
-                p = cfg_pos(a, a.o, d, tin, dt, k);
+                p = cfg_pos(a, go, gd, tin, dt, k);
                 const int cell = cell_of(a, p);
                 if (!a.cellempty[cell]) break;
                 const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
@@ -863,7 +860,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
                 if (qk > (float)(k + 1)) {
                     const int cand = qk < (float)n ? (int)qk : n;
                     if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, a.o, d, tin, dt, cand - 1);
+                        const Pos pv = cfg_pos(a, go, gd, tin, dt, cand - 1);
                         if (cell_of(a, pv) == cell) kc = cand;
                     }
                 }
@@ -873,7 +870,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
             if (k >= n) live = false;
         }
         if (live) {
-            if (!SKIP) p = cfg_pos(a, a.o, d, tin, dt, k); /* with SKIP the walk above left it at k */
+            if (!SKIP) p = cfg_pos(a, go, gd, tin, dt, k); /* with SKIP the walk above left it at k */
             const int bx = p.j >> a.bshift, by = p.i >> a.bshift, bz = p.k >> a.bshift;
             b = bx + a.nb[0] * (by + a.nb[1] * bz);
             x[0] = ((float)p.j + p.fx - (float)(bx << a.bshift)) * invB;

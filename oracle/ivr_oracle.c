@@ -227,7 +227,8 @@ static inline float lerpf_(float a, float b, float t) { return fmaf(t, b - a, a)
 
 /* Config-mode sampler: the reference's index mapping (ray.c:126-135) with the
  * weights it intends, f = u - floor(u), each on its own axis, clamp-to-edge.
- * u = (p - min) * (N / ext), clamped to [0, N-1]; j = min((int)u, N-2). */
+ * u = voxel coordinate (the caller's fmaf(dir*N/ext, t, (eye-min)*N/ext)),
+ * clamped to [0, N-1]; j = min((int)u, N-2). */
 static inline float tri_fetch(const float *vol, int nx, int ny, int nz, float gx, float gy,
                               float gz)
 {
@@ -646,13 +647,16 @@ static col4 ray_config(const rctx *c, const float o[3], const float d[3], long l
     float sx = (float)c->nx / (c->bmax[0] - c->bmin[0]);
     float sy = (float)c->ny / (c->bmax[1] - c->bmin[1]);
     float sz = (float)c->nz / (c->bmax[2] - c->bmin[2]);
+    /* the ray in voxel coordinates: g(t) = gd t + go */
+    const float gox = (o[0] - c->bmin[0]) * sx, goy = (o[1] - c->bmin[1]) * sy, goz = (o[2] - c->bmin[2]) * sz;
+    const float gdx = d[0] * sx, gdy = d[1] * sy, gdz = d[2] * sz;
     const int nx = c->nx, ny = c->ny, nz = c->nz;
     long long cnt = 0;
     for (int k = 0; k < n; k++) {
         float t = fmaf((float)k, dt, tin);
-        float gx = (fmaf(d[0], t, o[0]) - c->bmin[0]) * sx;
-        float gy = (fmaf(d[1], t, o[1]) - c->bmin[1]) * sy;
-        float gz = (fmaf(d[2], t, o[2]) - c->bmin[2]) * sz;
+        float gx = fmaf(gdx, t, gox);
+        float gy = fmaf(gdy, t, goy);
+        float gz = fmaf(gdz, t, goz);
         if (p->brick > 0) {
             float cx = fminf(fmaxf(gx, 0.0f), (float)(nx - 1));
             float cy = fminf(fmaxf(gy, 0.0f), (float)(ny - 1));
