@@ -38,17 +38,17 @@ def _oracle_sort_last(host, st, lut, B, rows=None):
     """oracle tiles of every brick (near to far by the sum of axis distances
     to the eye's brick, ties by id) composited by compositetiles"""
     import ctypes as C
-    n = host.shape[0]
+    dims = (host.shape[2], host.shape[1], host.shape[0])
     v = oracle.view_of(st)
     cam = np.zeros(23, np.float32)
     rect = (C.c_int * 4)()
-    bmin, bmax = oracle.box(n, n, n)
+    bmin, bmax = oracle.box(*dims)
     fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
     oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
-    eb = [int(np.floor((cam[r] - bmin[r]) * (n / (bmax[r] - bmin[r])) / B)) for r in range(3)]
-    nb = (n - 2) // B + 1
-    ids = sorted([(x, y, z) for z in range(nb) for y in range(nb) for x in range(nb)],
-                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb * (b[1] + nb * b[2])))
+    eb = [int(np.floor((cam[r] - bmin[r]) * (dims[r] / (bmax[r] - bmin[r])) / B)) for r in range(3)]
+    nb = [(dims[r] - 2) // B + 1 for r in range(3)]
+    ids = sorted([(x, y, z) for z in range(nb[2]) for y in range(nb[1]) for x in range(nb[0])],
+                 key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb[0] * (b[1] + nb[1] * b[2])))
     tiles, s_ref = [], 0
     for b in ids:
         t, s = oracle.render(host, v, lut, oracle.params(brick=B, brick_id=b), rows=rows)
@@ -195,6 +195,50 @@ def test_sort_last_matches_oracle_tiles(ctx, composite, B):
     f8, f32, _ = _render(ctx, vol, lut, ivr.config_desc(st), 96, 80)
     vol.close()
     assert np.abs(f32 - img32).max() <= 0.0101
+
+
+@pytest.mark.parametrize("B", [16, 32])
+def test_ragged_sizes_match_oracle(ctx, B):
+    """a 53x37x29 volume (no axis a multiple of the brick; bricks partly
+    filled, clamped tails) into a 101x67 frame (partial pixel blocks), fused
+    and sort-last, skipping on and off, bit for bit against the oracle"""
+    gp = configs.gen_params(2)
+    nx, ny, nz = 53, 37, 29
+    host = ivr.generate_host(gp, nx, ny, nz)
+    lut = configs.tf_lut(2)
+    W, H = 101, 67
+    st = configs.state(2, W, H)
+    vol = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=B)
+    vol.upload(host)
+    assert vol.info()["nbricks"] == tuple((d - 2) // B + 1 for d in (nx, ny, nz))
+    ref32, s_ref = oracle.render(host, oracle.view_of(st), lut, oracle.params())
+    f8, f32, fs = _render(ctx, vol, lut, ivr.config_desc(st), W, H)
+    _report("ragged fused B=%d" % B, f8, ref32, f32)
+    assert np.array_equal(f32.view(np.uint32), ref32.view(np.uint32)) and fs["samples"] == s_ref
+    k8, k32, ks = _render(ctx, vol, lut, ivr.config_desc(st, skip=True), W, H)
+    assert np.array_equal(k32.view(np.uint32), f32.view(np.uint32)) and ks["samples"] == s_ref
+    sl32, s_sl = _oracle_sort_last(host, st, lut, B)
+    for skip in (False, True):
+        l8, l32, ls = _render(ctx, vol, lut, ivr.config_desc(st, skip=skip, composite=ivr.SORT_LAST), W, H)
+        _report("ragged sort-last B=%d skip=%d" % (B, skip), l8, sl32, l32)
+        assert np.array_equal(l32.view(np.uint32), sl32.view(np.uint32)) and ls["samples"] == s_sl
+    vol.close()
+
+
+def test_minimum_sizes(ctx):
+    """the smallest volume (2^3) and frames of one pixel and one row"""
+    gp = configs.gen_params(1)
+    host = ivr.generate_host(gp, 2, 2, 2)
+    lut = configs.tf_lut(1)
+    vol = ctx.volume(2, 2, 2, ivr.IVR_F32, brick=16)
+    vol.upload(host)
+    for (W, H) in ((1, 1), (33, 1), (1, 17)):
+        st = configs.state(1, W, H)
+        for comp in (ivr.FUSED, ivr.SORT_LAST):
+            img8, img32, s = _render(ctx, vol, lut, ivr.config_desc(st, composite=comp), W, H)
+            ref32, s_ref = oracle.render(host, oracle.view_of(st), lut, oracle.params())
+            assert np.array_equal(img32.view(np.uint32), ref32.view(np.uint32)) and s["samples"] == s_ref, (W, H)
+    vol.close()
 
 
 def test_c2_full_size_rows_match_oracle(ctx):
