@@ -77,7 +77,11 @@ int ivr_ctx_create(int device, ivr_ctx **out)
     }
     if (const char *e = getenv("IVR_WAVE_W")) {
         const int w = atoi(e);
-        c->wave_w = (w == 4 || w == 16) ? w : 8;
+        c->wave_w = (w == 4 || w == 8 || w == 16 || w == 32 || w == 64) ? w : 16;
+    }
+    if (const char *e = getenv("IVR_BLOCK_W")) {
+        const int w = atoi(e);
+        c->block_w = (w == 32 || w == 64) ? w : 16;
     }
     IVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) IVR_HIP(hipEventCreate(&ev));

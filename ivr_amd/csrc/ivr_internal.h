@@ -17,8 +17,11 @@
 #define IVR_MAX_LUT 4096
 #define IVR_MAX_BRICKS 4096
 #define IVR_MAX_STEPS (1 << 22) /* per-ray sample bound */
-#define IVR_BLOCK 256 /* 4 wavefronts per 16x16 pixel block */
-#define IVR_SL_LDS 16384 /* LDS per workgroup for sort-last partials (4 runs) */
+#define IVR_BLOCK 256 /* threads of the batch and generation kernels */
+#ifndef IVR_FRAME_BLOCK
+#define IVR_FRAME_BLOCK 256 /* threads of a frame kernel's workgroup (one pixel block) */
+#endif
+#define IVR_SL_LDS (64 * IVR_FRAME_BLOCK) /* LDS for sort-last partials: 4 runs */
 
 int ivr_fail(int code, const char *fmt, ...);
 #define IVR_HIP(expr)                                                                          \
@@ -35,10 +38,11 @@ int ivr_fail(int code, const char *fmt, ...);
 
 struct ivr_ctx {
     int device = 0;
-    /* launch-shape tuning, read from IVR_XCD_CHUNK / IVR_WAVE_W at creation
+    /* launch-shape tuning, read from IVR_XCD_CHUNK / IVR_WAVE_W / IVR_BLOCK_W at creation
      * (results are identical for every setting) */
     int xcd_chunk = IVR_CHUNK_ROW;
     int wave_w = 16;
+    int block_w = 16; /* pixel block of a workgroup: block_w x (256 / block_w) */
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
     unsigned long long bytes = 0;
@@ -118,6 +122,7 @@ struct RenderArgs {
     int nblk_x, nblk, nblk_pad;
     int xcd_chunk;  /* blocks per XCD chunk, 0 = hardware order */
     int wave_shift; /* log2 of a wavefront's pixel width          */
+    int blk_shift;  /* log2 of a workgroup block's pixel width    */
     /* sort-last tile pass */
     int tb[3];
     int trect[4]; /* x0, y0, w, h */
@@ -152,7 +157,7 @@ __host__ __device__ static inline void brick_coords(int P, unsigned e, int l[3])
 void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
                    const ivr_frame_desc *d, ivr_frame *f);
 
-/* grid of 16x16 pixel blocks over a w x h frame with the context's chunking */
+/* grid of pixel blocks (block_w x 256/block_w) over a w x h frame with the context's chunking */
 void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h);
 
 /* launchers (render.hip) */

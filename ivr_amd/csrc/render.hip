@@ -33,7 +33,7 @@ using namespace ivr_dev;
 /* SL: 0 fused, 1 sort-last with the partials in LDS, 2 sort-last walking
  * the runs from the far end */
 template <int MODE, typename VT, bool SKIP, int SL>
-__global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render(const RenderArgs a)
 {
     extern __shared__ float4 smem[];
     float4 *s_lut = smem;
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render(const RenderArgs a)
  * its pixel rectangle, composited front to back from zero with ERT inside
  * the brick, into a premultiplied RGBA32F tile. */
 template <typename VT>
-__global__ __launch_bounds__(IVR_BLOCK) void k_tile(const RenderArgs a)
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_tile(const RenderArgs a)
 {
     extern __shared__ float4 smem[];
     float4 *s_lut = smem;
@@ -202,13 +202,13 @@ __global__ void k_cells(const float2 *range, unsigned char *empty, size_t ncells
 size_t smem_bytes(const RenderArgs &a, int runs = 0)
 {
     return (size_t)a.lutsize * sizeof(float4) + (size_t)((a.nbricks + 1) / 2) * sizeof(float4) +
-           (size_t)runs * IVR_BLOCK * sizeof(float4);
+           (size_t)runs * IVR_FRAME_BLOCK * sizeof(float4);
 }
 
 template <int MODE, typename VT, bool SKIP, int SL = 0>
 void launch_one(const RenderArgs &a, hipStream_t s, int runs = 0)
 {
-    hipLaunchKernelGGL((k_render<MODE, VT, SKIP, SL>), dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a, runs),
+    hipLaunchKernelGGL((k_render<MODE, VT, SKIP, SL>), dim3(a.nblk_pad), dim3(IVR_FRAME_BLOCK), smem_bytes(a, runs),
                        s, a);
 }
 
@@ -216,15 +216,18 @@ void launch_one(const RenderArgs &a, hipStream_t s, int runs = 0)
 
 void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h)
 {
-    a.nblk_x = (w + 15) / 16;
-    a.nblk = a.nblk_x * ((h + 15) / 16);
+    const int bw = ctx->block_w, bh = IVR_FRAME_BLOCK / bw;
+    a.blk_shift = bw == 64 ? 6 : bw == 32 ? 5 : 4;
+    a.nblk_x = (w + bw - 1) / bw;
+    a.nblk = a.nblk_x * ((h + bh - 1) / bh);
     int c = ctx->xcd_chunk;
     if (c == IVR_CHUNK_ROW) c = a.nblk_x;
     else if (c == IVR_CHUNK_BAND) c = (a.nblk + 7) / 8;
     a.xcd_chunk = c > 0 ? c : 0;
     const int pad = 8 * (c > 0 ? c : 1);
     a.nblk_pad = (a.nblk + pad - 1) / pad * pad;
-    a.wave_shift = ctx->wave_w == 4 ? 2 : ctx->wave_w == 16 ? 4 : 3;
+    a.wave_shift = ctx->wave_w == 4 ? 2 : ctx->wave_w == 16 ? 4 : ctx->wave_w == 32 ? 5 : ctx->wave_w == 64 ? 6 : 3;
+    if (a.wave_shift > a.blk_shift) a.wave_shift = a.blk_shift;
 }
 
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last)
@@ -238,7 +241,7 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
     } else if (sort_last) {
         /* partials in LDS while they fit in IVR_SL_LDS bytes per workgroup */
         const int runs = a.nb[0] + a.nb[1] + a.nb[2] - 2;
-        if ((size_t)runs * IVR_BLOCK * sizeof(float4) <= IVR_SL_LDS && smem_bytes(a, runs) <= 65536) {
+        if ((size_t)runs * IVR_FRAME_BLOCK * sizeof(float4) <= IVR_SL_LDS && smem_bytes(a, runs) <= 65536) {
             if (f16) {
                 if (skip) launch_one<IVR_MODE_CONFIG, h16, true, 1>(a, s, runs);
                 else launch_one<IVR_MODE_CONFIG, h16, false, 1>(a, s, runs);
@@ -268,9 +271,9 @@ int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a)
 {
     ivr_set_grid(a, ctx, a.trect[2], a.trect[3]);
     if (vol->dtype == IVR_F16)
-        hipLaunchKernelGGL(k_tile<h16>, dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a), ctx->stream, a);
+        hipLaunchKernelGGL(k_tile<h16>, dim3(a.nblk_pad), dim3(IVR_FRAME_BLOCK), smem_bytes(a), ctx->stream, a);
     else
-        hipLaunchKernelGGL(k_tile<float>, dim3(a.nblk_pad), dim3(IVR_BLOCK), smem_bytes(a), ctx->stream, a);
+        hipLaunchKernelGGL(k_tile<float>, dim3(a.nblk_pad), dim3(IVR_FRAME_BLOCK), smem_bytes(a), ctx->stream, a);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }

@@ -560,12 +560,13 @@ __device__ __forceinline__ void stage(const RenderArgs &a, float4 *s_lut, unsign
     __syncthreads();
 }
 
-/* Workgroup -> 16x16 pixel block; returns false for padding.  Workgroups
+/* Workgroup -> pixel block (2^blk_shift wide, IVR_FRAME_BLOCK pixels);
+ * returns false for padding.  Workgroups
  * are dealt to the 8 XCDs round robin (XCD = blockIdx % 8).  With a chunk
  * C > 0 each XCD renders runs of C consecutive blocks, so one L2 sees
  * neighbouring blocks, while the runs stay interleaved over the frame so
  * every XCD gets a share of the expensive centre.  Inside the block each
- * of the four wavefronts covers 2^s x 2^(6-s) pixels, s = wave_shift. */
+ * wavefront covers 2^s x 2^(6-s) pixels, s = wave_shift. */
 __device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
 {
     const int b = blockIdx.x;
@@ -577,9 +578,9 @@ __device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
     if (L >= a.nblk) return false;
     const int bx = L % a.nblk_x, by = L / a.nblk_x;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int s = a.wave_shift, ww = 1 << s, per_row = 16 >> s;
-    j = bx * 16 + (w % per_row) * ww + (lane & (ww - 1));
-    i = by * 16 + (w / per_row) * (64 >> s) + (lane >> s);
+    const int s = a.wave_shift, ww = 1 << s, per_row = (1 << a.blk_shift) >> s;
+    j = (bx << a.blk_shift) + (w % per_row) * ww + (lane & (ww - 1));
+    i = by * (IVR_FRAME_BLOCK >> a.blk_shift) + (w / per_row) * (64 >> s) + (lane >> s);
     return true;
 }
 

@@ -802,7 +802,7 @@ struct SgRender {
  * of transparent macrocells (the same exact jumps as the plain renderer),
  * so only samples that can contribute reach the network. */
 template <bool SKIP>
-__global__ __launch_bounds__(IVR_BLOCK) void k_render_sg(const SgRender S)
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
 {
     extern __shared__ float4 smem[];
     const RenderArgs &a = S.a;
@@ -980,8 +980,8 @@ int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const
         if (rc) return rc;
     }
     const size_t lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
-    if (skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.a.nblk_pad), dim3(IVR_BLOCK), lds, ctx->stream, S);
-    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.a.nblk_pad), dim3(IVR_BLOCK), lds, ctx->stream, S);
+    if (skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, ctx->stream, S);
+    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, ctx->stream, S);
     IVR_HIP(hipGetLastError());
     IVR_HIP(hipEventRecord(fr->e[1], ctx->stream));
     return IVR_OK;
