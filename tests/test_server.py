@@ -18,6 +18,7 @@ from ivr_amd import configs
 
 pytestmark = pytest.mark.gpu
 SERVER = os.path.join(os.path.dirname(ivr.LIB_PATH), "ivrserver")
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def free_port():
@@ -129,6 +130,48 @@ def test_listen_mode_concurrent_clients(ctx, refs):
     assert len(results) == 12
     for key, (vol, st, lut, img) in results.items():
         assert np.array_equal(img, render_ref(ctx, refs[vol], st, lut)), key
+
+
+RELAY = os.path.join(ROOT, "oracle", "_ref", "myserver")
+
+
+@pytest.mark.skipif(not os.path.exists(RELAY), reason="reference relay not built (make -C oracle relay)")
+def test_behind_the_reference_relay(ctx, refs):
+    """the reference's own relay (myserver.c, compiled from its sources)
+    between a client and ivrserver: client first, render server second, as
+    the reference deploys them; frames equal the in-process renders"""
+    port = free_port()
+    relay = subprocess.Popen([RELAY, str(port)], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    srv = None
+    try:
+        cl = None
+        t0 = time.time()
+        while time.time() - t0 < 30:
+            try:
+                cl = socket.create_connection(("127.0.0.1", port), timeout=60)
+                break
+            except OSError:
+                time.sleep(0.1)
+        assert cl is not None, "relay did not listen"
+        assert recv_all(cl, 13) == b"Hello, world!"
+        cl.sendall(b"Hello, server!")
+        srv = start_server(["--relay", "127.0.0.1:%d" % port, "--volume", "gen:1:64", "--max", "256", "256",
+                            "--budget-gb", "1"], until="ivrserver: ready")
+        for seed in (3101, 3102, 3103):
+            st = ivr.orbit_state(seed, 112, 80)
+            cl.sendall(ivr.state_pack(st))
+            img = np.frombuffer(recv_all(cl, 112 * 80 * 4), np.uint8).reshape(80, 112, 4)
+            assert np.array_equal(img, render_ref(ctx, refs[0], st, configs.tf_lut(1))), seed
+        fin = ivr.State()
+        fin.finish = 1
+        cl.sendall(ivr.state_pack(fin))
+        assert srv.wait(timeout=60) == 0
+        cl.close()
+    finally:
+        relay.kill()  # the reference relay serves until it is stopped
+        relay.wait(timeout=30)
+        if srv is not None and srv.poll() is None:
+            srv.kill()
 
 
 def test_relay_mode_is_a_dropin_render_server(ctx, refs):
