@@ -132,6 +132,50 @@ def test_listen_mode_concurrent_clients(ctx, refs):
         assert np.array_equal(img, render_ref(ctx, refs[vol], st, lut)), key
 
 
+def test_raw_big_endian_volume(ctx, tmp_path):
+    """--volume raw: a big-endian float32 file of arbitrary dims and value
+    range (as readvoldata reads the reference's datasets, main.c:629-706),
+    LUT spanning the data range; frames equal the in-process renders"""
+    nx, ny, nz = 61, 47, 53
+    blobs = ivr.generate_host(configs.gen_params(1), nx, ny, nz)
+    host = (np.float32(1468.0) + np.float32(3518.0) * blobs).astype(np.float32)
+    path = str(tmp_path / "shell.raw")
+    host.astype(">f4").tofile(path)
+    port = free_port()
+    srv = start_server(["--listen", str(port), "--volume", "raw:%s:%d:%d:%d:0" % (path, nx, ny, nz),
+                        "--max", "200", "200", "--budget-gb", "1"])
+    try:
+        lo, hi = np.float32(host.min()), np.float32(host.max())
+        scale = np.float32(255.0) / (hi - lo)
+        vol = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=64)
+        vol.upload(host)
+        tf = ctx.tf(configs.tf_lut(2))
+        s = socket.create_connection(("127.0.0.1", port))
+        assert recv_all(s, 13) == b"Hello, world!"
+        s.sendall(b"Hello, server!")
+        for seed in (3201, 3202):
+            st = ivr.orbit_state(seed, 150, 110)
+            s.sendall(ivr.state_pack(st))
+            img = np.frombuffer(recv_all(s, 150 * 110 * 4), np.uint8).reshape(110, 150, 4)
+            d = ivr.config_desc(st, skip=True, lutscale=float(scale))
+            d.lutoffset = int(np.float32(lo * scale))
+            fr = ctx.frame(150, 110)
+            ctx.render(vol, tf, d, fr)
+            ref = fr.download()
+            fr.close()
+            assert np.array_equal(img, ref), seed
+            assert (ref[..., 3] > 0).mean() > 0.05
+        fin = ivr.State()
+        fin.finish = 1
+        s.sendall(ivr.state_pack(fin))
+        s.close()
+        tf.close()
+        vol.close()
+    finally:
+        srv.terminate()
+        srv.wait(timeout=30)
+
+
 RELAY = os.path.join(ROOT, "oracle", "_ref", "myserver")
 
 
