@@ -14,6 +14,7 @@
 #include "ivr_host.h"
 
 #define IVR_CELL_SHIFT 4 /* macrocells of 16^3 voxels */
+#define IVR_MIP_LEVELS 3 /* coarser transparent nodes: 2^3, 4^3, 8^3 macrocells */
 #define IVR_MAX_LUT 4096
 #define IVR_MAX_BRICKS 4096
 #define IVR_MAX_STEPS (1 << 22) /* per-ray sample bound */
@@ -64,7 +65,10 @@ struct ivr_volume {
     unsigned long long *d_boff = nullptr;
     int nc[3] = {0, 0, 0};              /* macrocells per axis                  */
     float2 *d_cellrange = nullptr;      /* per-cell value min/max (with footprint) */
-    unsigned char *d_cellempty = nullptr; /* per-cell "all samples transparent"  */
+    /* per-cell transparency code of the current frame: 0 = some sample may
+     * be visible, else 1 + the largest level L <= IVR_MIP_LEVELS whose node
+     * of (2^L)^3 cells around the cell is all transparent */
+    unsigned char *d_cellempty = nullptr;
     size_t ncells = 0;
 };
 
@@ -163,6 +167,7 @@ void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h);
 /* launchers (render.hip) */
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last);
 int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a);
+/* transparency codes (see ivr_volume::d_cellempty) of the frame's LUT into out, ncells bytes */
 int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out);
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a);
 int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles);

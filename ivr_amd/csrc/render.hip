@@ -173,17 +173,20 @@ __global__ __launch_bounds__(256) void k_composite(const float4 *tiles, const Ti
     if (out32) out32[px] = d;
 }
 
-/* macrocell transparency for this frame's LUT: a cell is empty when every
- * LUT entry its samples can index has zero opacity.  A sample is a chain of
- * three lerps of the cell's voxels, so it lies in [min, max] up to a few
- * ulps; the range is widened by 2^-20 relative (8 ulps) plus 2^-20 absolute
- * before the monotone index map (int)(s * lutscale) - lutoffset. */
-__global__ void k_cells(const float2 *range, unsigned char *empty, size_t ncells, const float4 *lut,
-                        int lutsize, float lutscale, int lutoffset, float alpha)
+/* Macrocell transparency for this frame's LUT: a cell is transparent when
+ * every LUT entry its samples can index has zero opacity.  A sample is a
+ * chain of three lerps of the cell's voxels, so it lies in [min, max] up to
+ * a few ulps; the range is widened by 2^-20 relative (8 ulps) plus 2^-20
+ * absolute before the monotone index map (int)(s * lutscale) - lutoffset.
+ *
+ * One workgroup per node of 8^3 cells: the cells' flags go to LDS and are
+ * reduced to the nodes of 2^3, 4^3 and 8^3 cells containing them (cells
+ * past the volume's extent hold no samples and count as transparent).  The
+ * code written per cell is 0 when it may hold a visible sample, else 1 +
+ * the largest level whose node around it is transparent. */
+__device__ __forceinline__ bool cell_transparent(float2 r, const float4 *lut, int lutsize, float lutscale,
+                                                 int lutoffset, float alpha)
 {
-    const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const float2 r = range[c];
     const float tol = 9.5367431640625e-07f; /* 2^-20 */
     float lo_s = r.x * lutscale, hi_s = r.y * lutscale;
     lo_s = lo_s - fabsf(lo_s) * tol - tol;
@@ -193,10 +196,56 @@ __global__ void k_cells(const float2 *range, unsigned char *empty, size_t ncells
     int lo = (int)lo_s - lutoffset, hi = (int)hi_s - lutoffset;
     lo = min(max(lo, 0), lutsize - 1);
     hi = min(max(hi, 0), lutsize - 1);
-    unsigned char e = 1;
     for (int x = lo; x <= hi; x++)
-        if (lut[x].w * alpha != 0.0f) { e = 0; break; }
-    empty[c] = e;
+        if (lut[x].w * alpha != 0.0f) return false;
+    return true;
+}
+
+__global__ __launch_bounds__(512) void k_cells(const float2 *range, unsigned char *code, int ncx, int ncy,
+                                               int ncz, const float4 *lut, int lutsize, float lutscale,
+                                               int lutoffset, float alpha)
+{
+    __shared__ unsigned char s0[512], s1[64], s2[8], s3;
+    const int t = threadIdx.x;
+    const int tx = t & 7, ty = (t >> 3) & 7, tz = t >> 6;
+    const int x = blockIdx.x * 8 + tx, y = blockIdx.y * 8 + ty, z = blockIdx.z * 8 + tz;
+    const bool in = x < ncx && y < ncy && z < ncz;
+    const size_t c = (size_t)x + (size_t)ncx * ((size_t)y + (size_t)ncy * z);
+    const bool e = !in || cell_transparent(range[in ? c : 0], lut, lutsize, lutscale, lutoffset, alpha);
+    s0[t] = e;
+    __syncthreads();
+    if (t < 64) {
+        const int ux = t & 3, uy = (t >> 2) & 3, uz = t >> 4;
+        unsigned char m = 1;
+        for (int q = 0; q < 8; q++)
+            m &= s0[(2 * ux + (q & 1)) + 8 * ((2 * uy + ((q >> 1) & 1)) + 8 * (2 * uz + (q >> 2)))];
+        s1[t] = m;
+    }
+    __syncthreads();
+    if (t < 8) {
+        const int ux = t & 1, uy = (t >> 1) & 1, uz = t >> 2;
+        unsigned char m = 1;
+        for (int q = 0; q < 8; q++)
+            m &= s1[(2 * ux + (q & 1)) + 4 * ((2 * uy + ((q >> 1) & 1)) + 4 * (2 * uz + (q >> 2)))];
+        s2[t] = m;
+    }
+    __syncthreads();
+    if (t == 0) {
+        unsigned char m = 1;
+        for (int q = 0; q < 8; q++) m &= s2[q];
+        s3 = m;
+    }
+    __syncthreads();
+    if (!in) return;
+    unsigned char v = 0;
+    if (e) {
+        v = 1;
+        if (s1[(tx >> 1) + 4 * ((ty >> 1) + 4 * (tz >> 1))]) {
+            v = 2;
+            if (s2[(tx >> 2) + 2 * ((ty >> 2) + 2 * (tz >> 2))]) v = s3 ? 4 : 3;
+        }
+    }
+    code[c] = v;
 }
 
 size_t smem_bytes(const RenderArgs &a, int runs = 0)
@@ -285,9 +334,10 @@ int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a)
 
 int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out)
 {
-    const unsigned nb = (unsigned)((vol->ncells + 255) / 256);
-    hipLaunchKernelGGL(k_cells, dim3(nb), dim3(256), 0, stream, vol->d_cellrange, out, vol->ncells, a.lut,
-                       a.lutsize, a.lutscale, a.lutoffset, a.alpha);
+    static_assert(IVR_MIP_LEVELS == 3, "k_cells reduces nodes of 8^3 cells");
+    const dim3 g((vol->nc[0] + 7) / 8, (vol->nc[1] + 7) / 8, (vol->nc[2] + 7) / 8);
+    hipLaunchKernelGGL(k_cells, g, dim3(512), 0, stream, vol->d_cellrange, out, vol->nc[0], vol->nc[1],
+                       vol->nc[2], a.lut, a.lutsize, a.lutscale, a.lutoffset, a.alpha);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }

@@ -134,6 +134,39 @@ __device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
            a.nc[0] * ((p.i >> IVR_CELL_SHIFT) + a.nc[1] * (p.k >> IVR_CELL_SHIFT));
 }
 
+/* First sample index past k on the grid t_k = tin + k dt whose cell may be
+ * visible, given that sample k at p has the nonzero transparency code e
+ * (its node of 2^(e-1) cells per axis is transparent): the sample just
+ * before the ray's exit from that node, estimated from the exit faces, is
+ * checked to lie in the node, else the walk advances by one sample.  Each
+ * axis index of the samples is monotone in k, so all samples between lie
+ * in the node.  The result is at most kmax. */
+__device__ __forceinline__ int skip_empty(const RenderArgs &a, const float *go, const float *gd,
+                                          const float *rgd, float tin, float dt, const Pos &p, int e,
+                                          int k, int kmax)
+{
+    const int sh = IVR_CELL_SHIFT - 1 + e;
+    const int cc[3] = {p.j >> sh, p.i >> sh, p.k >> sh};
+    float te = FLT_MAX;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (gd[q] != 0.0f) {
+            const float face = (float)((cc[q] + (gd[q] > 0.0f ? 1 : 0)) << sh);
+            te = fminf(te, (face - go[q]) * rgd[q]);
+        }
+    }
+    int kc = k + 1;
+    const float qk = (te - tin) / dt;
+    if (qk > (float)(k + 1)) {
+        const int cand = qk < (float)kmax ? (int)qk : kmax;
+        if (cand > k + 1) {
+            const Pos pv = cfg_pos(a, go, gd, tin, dt, cand - 1);
+            if ((((pv.j ^ p.j) | (pv.i ^ p.i) | (pv.k ^ p.k)) >> sh) == 0) kc = cand;
+        }
+    }
+    return kc;
+}
+
 /* per-pixel config-mode ray direction: M (u, v, 5) */
 __device__ __forceinline__ void cfg_dir(const RenderArgs &a, int i, int j, float *d)
 {
@@ -198,32 +231,18 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
 {
     int k = k0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
+    int cc = -1, e = 0;            /* macrocell of the code held in e      */
     float v[8];
     while (k < k1) {
         const Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
         if (SKIP) {
             const int cell = cell_of(a, p);
-            if (a.cellempty[cell]) {
-                /* every sample of this cell is transparent: jump to the first
-                 * sample past the cell exit, verified exactly on the grid */
-                const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
-                float te = FLT_MAX;
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    if (r.gd[q] != 0.0f) {
-                        const float face = (float)((cc[q] + (r.gd[q] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
-                        te = fminf(te, (face - r.go[q]) * r.rgd[q]);
-                    }
-                }
-                int kc = k + 1;
-                const float qk = (te - r.tin) / r.dt;
-                if (qk > (float)(k + 1)) {
-                    const int cand = qk < (float)k1 ? (int)qk : k1;
-                    if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, r.go, r.gd, r.tin, r.dt, cand - 1);
-                        if (cell_of(a, pv) == cell) kc = cand;
-                    }
-                }
+            if (cell != cc) { /* the code is loaded once per macrocell visited */
+                cc = cell;
+                e = a.cellempty[cell];
+            }
+            if (e) { /* transparent node: jump past it */
+                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, p, e, k, k1);
                 cnt += (unsigned)(kc - k);
                 k = kc;
                 continue;
@@ -340,6 +359,7 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
     int k = 0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
+    int cc = -1, e = 0;            /* macrocell of the code held in e      */
     float v[8];
     while (k < r.n) {
         const Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
@@ -351,27 +371,15 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
         }
         if (SKIP) {
             const int cell = cell_of(a, p);
-            if (a.cellempty[cell]) {
-                /* transparent cell (never wider than a brick): jump to the
-                 * first sample past its exit, verified exactly on the grid */
-                const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
-                float te = FLT_MAX;
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    if (r.gd[q] != 0.0f) {
-                        const float face = (float)((cc[q] + (r.gd[q] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
-                        te = fminf(te, (face - r.go[q]) * r.rgd[q]);
-                    }
-                }
-                int kc = k + 1;
-                const float qk = (te - r.tin) / r.dt;
-                if (qk > (float)(k + 1)) {
-                    const int cand = qk < (float)r.n ? (int)qk : r.n;
-                    if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, r.go, r.gd, r.tin, r.dt, cand - 1);
-                        if (cell_of(a, pv) == cell) kc = cand;
-                    }
-                }
+            if (cell != cc) {
+                cc = cell;
+                e = a.cellempty[cell];
+            }
+            if (e) {
+                /* transparent node: jump past it.  Its samples add exact
+                 * zeros to the run's partial, and a run skipped whole has an
+                 * all-zero partial, which `over` leaves without effect. */
+                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, p, e, k, r.n);
                 cnt += (unsigned)(kc - k);
                 k = kc;
                 continue;

@@ -837,6 +837,7 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
 #pragma unroll
         for (int q = 0; q < 3; q++) rgd[q] = 1.0f / gd[q];
     }
+    int cc = -1, e = 0; /* macrocell of the transparency code held in e */
     while (__ballot(live) != 0ull) {
         float x[3] = {0.5f, 0.5f, 0.5f};
         int b = 0;
@@ -845,25 +846,12 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
             while (k < n) {
                 p = cfg_pos(a, go, gd, tin, dt, k);
                 const int cell = cell_of(a, p);
-                if (!a.cellempty[cell]) break;
-                const int cc[3] = {p.j >> IVR_CELL_SHIFT, p.i >> IVR_CELL_SHIFT, p.k >> IVR_CELL_SHIFT};
-                float te = FLT_MAX;
-#pragma unroll
-                for (int q = 0; q < 3; q++) {
-                    if (gd[q] != 0.0f) {
-                        const float face = (float)((cc[q] + (gd[q] > 0.0f ? 1 : 0)) << IVR_CELL_SHIFT);
-                        te = fminf(te, (face - go[q]) * rgd[q]);
-                    }
+                if (cell != cc) {
+                    cc = cell;
+                    e = a.cellempty[cell];
                 }
-                int kc = k + 1;
-                const float qk = (te - tin) / dt;
-                if (qk > (float)(k + 1)) {
-                    const int cand = qk < (float)n ? (int)qk : n;
-                    if (cand > k + 1) {
-                        const Pos pv = cfg_pos(a, go, gd, tin, dt, cand - 1);
-                        if (cell_of(a, pv) == cell) kc = cand;
-                    }
-                }
+                if (!e) break;
+                const int kc = skip_empty(a, go, gd, rgd, tin, dt, p, e, k, n);
                 cnt += (unsigned)(kc - k);
                 k = kc;
             }
@@ -975,6 +963,7 @@ int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const
     if (skip) {
         ivr_volume cells; /* the surrogate's own macrocell ranges */
         cells.ncells = s->ncells;
+        for (int r = 0; r < 3; r++) cells.nc[r] = s->nc[r];
         cells.d_cellrange = s->d_cellrange;
         int rc = ivr_launch_cells_on(ctx->stream, &cells, S.a, s->d_cellempty);
         if (rc) return rc;

@@ -197,6 +197,52 @@ def test_sort_last_matches_oracle_tiles(ctx, composite, B):
     assert np.abs(f32 - img32).max() <= 0.0101
 
 
+def _sparse_volume(nx, ny, nz, seed):
+    """zeros with a few gaussian blobs: large transparent regions, so the
+    skip jumps over nodes of up to 8^3 macrocells"""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    vol = np.zeros((nz, ny, nx), np.float32)
+    for _ in range(6):
+        c = rng.uniform(0.15, 0.85, 3) * (nx, ny, nz)
+        r2 = ((x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2) / rng.uniform(200.0, 600.0)
+        vol += np.exp(-r2).astype(np.float32)
+    return np.minimum(vol, 1.0).astype(np.float32)
+
+
+def _sparse_lut():
+    lut = np.zeros((256, 4), np.float32)
+    t = np.linspace(0.0, 1.0, 256, dtype=np.float32)
+    lut[:, 0], lut[:, 1], lut[:, 2] = t, 1.0 - t, 0.5 + 0.5 * t
+    lut[100:, 3] = np.linspace(0.05, 0.6, 156, dtype=np.float32)
+    return lut
+
+
+@pytest.mark.parametrize("B", [32, 64])
+def test_hierarchical_skip_on_sparse_volume(ctx, B):
+    """a mostly transparent 160x144x176 volume: skipping over empty nodes of
+    1..8 macrocells per axis (crossing brick faces) is invisible, fused and
+    sort-last, bit for bit against the oracle, from several views"""
+    nx, ny, nz = 160, 144, 176
+    host = _sparse_volume(nx, ny, nz, 7)
+    lut = _sparse_lut()
+    W, H = 96, 88
+    vol = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=B)
+    vol.upload(host)
+    for seed in (11, 12, 13):
+        st = configs.state(2, W, H, seed=seed)
+        ref32, s_ref = oracle.render(host, oracle.view_of(st), lut, oracle.params())
+        f8, f32, fs = _render(ctx, vol, lut, ivr.config_desc(st, skip=True), W, H)
+        _report("sparse fused skip B=%d view %d" % (B, seed), f8, ref32, f32)
+        assert np.array_equal(f32.view(np.uint32), ref32.view(np.uint32)) and fs["samples"] == s_ref
+        assert fs["visited"] < fs["samples"] // 4
+        sl32, s_sl = _oracle_sort_last(host, st, lut, B)
+        l8, l32, ls = _render(ctx, vol, lut, ivr.config_desc(st, skip=True, composite=ivr.SORT_LAST), W, H)
+        _report("sparse sort-last skip B=%d view %d" % (B, seed), l8, sl32, l32)
+        assert np.array_equal(l32.view(np.uint32), sl32.view(np.uint32)) and ls["samples"] == s_sl
+    vol.close()
+
+
 @pytest.mark.parametrize("B", [16, 32])
 def test_ragged_sizes_match_oracle(ctx, B):
     """a 53x37x29 volume (no axis a multiple of the brick; bricks partly
