@@ -121,11 +121,21 @@ def main():
     log("%s: %d samples classified per frame (%d incl. skipped); %.3f ms/frame device, %.3f wall, "
         "%.1f Gsamples/s" % (comp_name, S, stats["samples"], dev_ms, wall_ms, gsps))
     img = fr.download()
+    # the dominant kernel's own launch duration, from the HIP events the frame
+    # records on the context's stream around it (and around the combine pass)
+    k_ms, c_ms = kernel_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), fr, args.steps)
     vox_bytes = 2 if cfg["dtype"] == ivr.IVR_F16 else 4
-    bytes_alg = S * 8 * vox_bytes + 4 * W * H  # sort-last partials stay in registers: no tile bytes
-    achieved = bytes_alg / (dev_ms * 1e-3)
-    kernel = "k_render<1, %s, %s, %s>" % ("float" if vox_bytes == 4 else "_Float16", str(skip).lower(),
-                                          str(comp == ivr.SORT_LAST).lower())
+    vt = "float" if vox_bytes == 4 else "_Float16"
+    if comp == ivr.SORT_LAST:
+        # one workgroup per (pixel block, run): 8 voxels per classified sample
+        # read, one RGBA32F partial plane per run written
+        runs = sum(vol.info()["nbricks"]) - 2
+        bytes_alg = S * 8 * vox_bytes + runs * W * H * 16
+        kernel = "k_runs<%s, %s>" % (vt, str(skip).lower())
+    else:
+        bytes_alg = S * 8 * vox_bytes + 4 * W * H
+        kernel = "k_render<1, %s, %s, 0>" % (vt, str(skip).lower())
+    achieved = bytes_alg / (k_ms * 1e-3)
     traffic = measured_traffic(cfg["name"], comp_name, skip)
 
     other = {}
@@ -170,6 +180,8 @@ def main():
         "roofline": {
             "bound": "hbm",
             "kernel": kernel,
+            "kernel_ms": round(k_ms, 4),
+            "combine_ms": round(c_ms, 4),
             "achieved": round(achieved / 1e9, 1),
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
@@ -186,6 +198,19 @@ def main():
     tf.close()
     vol.close()
     ctx.close()
+
+
+def kernel_ms(ctx, render, fr, steps):
+    """mean duration of the frame's render kernel and of its combine pass
+    (zero when there is none) over `steps` frames, each read back from the
+    frame's HIP events before the next frame is queued"""
+    r, c = [], []
+    for _ in range(steps):
+        render()
+        s = fr.stats()
+        r.append(s["render_ms"])
+        c.append(s["composite_ms"])
+    return sum(r) / len(r), sum(c) / len(c)
 
 
 def _frame_ms(ctx, fn, steps):

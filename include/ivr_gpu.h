@@ -101,8 +101,9 @@ typedef struct ivr_frame_desc {
 typedef struct ivr_frame_stats {
     unsigned long long samples; /* samples classified and composited           */
     unsigned long long visited; /* samples whose voxels were fetched (after skip) */
-    float render_ms;            /* device time of the last render (events)      */
-    float composite_ms;         /* device time of the IVR_SORT_LAST_TILES over pass */
+    float render_ms;            /* device time of the last render's kernel(s), after
+                                   the macrocell pass (events)                  */
+    float composite_ms;         /* device time of its sort-last combine pass    */
 } ivr_frame_stats;
 
 /* ---- context ---- */

@@ -83,6 +83,7 @@ int ivr_ctx_create(int device, ivr_ctx **out)
         const int w = atoi(e);
         c->block_w = (w == 32 || w == 64) ? w : 16;
     }
+    if (const char *e = getenv("IVR_SL_SPLIT")) c->sl_split = atoi(e) != 0;
     IVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) IVR_HIP(hipEventCreate(&ev));
     *out = c;
@@ -420,12 +421,29 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
     IVR_HIP(hipMemsetAsync(f->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
     f->timed = 1;
     f->composited = 0;
-    IVR_HIP(hipEventRecord(f->e[0], ctx->stream));
     int rc;
     const int skip = d->mode == IVR_MODE_CONFIG && d->skip;
     if (skip && (rc = ivr_launch_cells(ctx, v, a))) return rc;
+    IVR_HIP(hipEventRecord(f->e[0], ctx->stream)); /* render_ms: the render kernel(s) alone */
     if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST_TILES) {
         if ((rc = render_sort_last(ctx, v, d, cam, a, f))) return rc;
+        IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
+        return IVR_OK;
+    }
+    if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST && ctx->sl_split) {
+        const int runs = v->nb[0] + v->nb[1] + v->nb[2] - 2;
+        const size_t need = (size_t)runs * f->W * f->H;
+        if (need > f->tiles_cap) {
+            ivr_dfree(ctx, f->d_tiles, f->tiles_cap * sizeof(float4));
+            f->tiles_cap = 0;
+            if ((rc = ivr_dalloc(ctx, (void **)&f->d_tiles, need * sizeof(float4)))) return rc;
+            f->tiles_cap = need;
+        }
+        a.tile = f->d_tiles;
+        if ((rc = ivr_launch_runs(ctx, v, a, skip))) return rc;
+        IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
+        f->composited = 1;
+        if ((rc = ivr_launch_combine_runs(ctx, a, runs))) return rc;
         IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
         return IVR_OK;
     }

@@ -44,6 +44,7 @@ struct ivr_ctx {
     int xcd_chunk = IVR_CHUNK_ROW;
     int wave_w = 16;
     int block_w = 16; /* pixel block of a workgroup: block_w x (256 / block_w) */
+    int sl_split = 1; /* sort-last as one workgroup per (pixel block, run) */
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
     unsigned long long bytes = 0;
@@ -170,6 +171,9 @@ int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a);
 /* transparency codes (see ivr_volume::d_cellempty) of the frame's LUT into out, ncells bytes */
 int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out);
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a);
+/* sort-last with one workgroup per (pixel block, run): partial planes into a.tile, then combined */
+int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);
+int ivr_launch_combine_runs(ivr_ctx *ctx, const RenderArgs &a, int runs);
 int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles);
 
 /* volume kernels (volume.hip) */

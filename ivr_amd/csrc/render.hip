@@ -40,7 +40,7 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render(const RenderArgs a)
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     float4 *s_part = smem + a.lutsize + (a.nbricks + 1) / 2;
     int i, j;
-    if (!block_pixel(a, i, j)) return; /* uniform per workgroup */
+    if (!block_pixel(a, blockIdx.x, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
     const bool inside = (i < a.H) && (j < a.W);
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -72,6 +72,51 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render(const RenderArgs a)
     count_add(&a.counters[1], fetched);
 }
 
+/* Sort-last with every run of every ray as its own work: workgroup g
+ * renders run g / nblk_pad (counted front to back along each ray) of the
+ * rays of pixel block g % nblk_pad, composited from zero with early
+ * termination inside the run, into plane `run` of a.tile (W x H float4 per
+ * plane); rays without that run write zeros.  Splitting the rays at the
+ * brick faces shortens the longest chains of dependent samples, which
+ * otherwise leave most of the GPU idle in the second half of a frame. */
+template <typename VT, bool SKIP>
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
+{
+    extern __shared__ float4 smem[];
+    float4 *s_lut = smem;
+    unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
+    const int run = blockIdx.x / a.nblk_pad;
+    int i, j;
+    if (!block_pixel(a, blockIdx.x - run * a.nblk_pad, i, j)) return; /* uniform per workgroup */
+    stage(a, s_lut, s_boff);
+    unsigned cnt = 0, fetched = 0;
+    if (i < a.H && j < a.W) {
+        float d[3];
+        cfg_dir(a, i, j, d);
+        Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        march_run<VT, SKIP>(a, s_lut, s_boff, a.o, d, run, acc, cnt, fetched);
+        a.tile[((size_t)run * a.H + i) * a.W + j] = make_float4(acc.r, acc.g, acc.b, acc.a);
+    }
+    count_add(&a.counters[0], cnt);
+    count_add(&a.counters[1], fetched);
+}
+
+/* the planes of k_runs combined far to near per pixel, then quantised */
+__global__ __launch_bounds__(256) void k_combine_runs(const float4 *part, int runs, int W, int H, uchar4 *out8,
+                                                      float4 *out32)
+{
+    const size_t np = (size_t)W * H, px = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (px >= np) return;
+    Acc d = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int u = runs - 1; u >= 0; u--) {
+        const float4 t = part[u * np + px];
+        const Acc s = {t.x, t.y, t.z, t.w};
+        over(d, s);
+    }
+    out8[px] = make_uchar4(quant8(d.r), quant8(d.g), quant8(d.b), quant8(d.a));
+    if (out32) out32[px] = make_float4(d.r, d.g, d.b, d.a);
+}
+
 /* Sort-last pass for brick a.tb: the brick's owned samples of each ray of
  * its pixel rectangle, composited front to back from zero with ERT inside
  * the brick, into a premultiplied RGBA32F tile. */
@@ -82,7 +127,7 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_tile(const RenderArgs a)
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     int ti, tj;
-    if (!block_pixel(a, ti, tj)) return; /* uniform per workgroup */
+    if (!block_pixel(a, blockIdx.x, ti, tj)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
     const bool inside = (ti < a.trect[3]) && (tj < a.trect[2]);
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -312,6 +357,33 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
         if (f16) launch_one<IVR_MODE_CONFIG, h16, false>(a, s);
         else launch_one<IVR_MODE_CONFIG, float, false>(a, s);
     }
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip)
+{
+    ivr_set_grid(a, ctx, a.W, a.H);
+    const int runs = a.nb[0] + a.nb[1] + a.nb[2] - 2;
+    const dim3 g((unsigned)a.nblk_pad * (unsigned)runs);
+    const size_t sm = smem_bytes(a);
+    hipStream_t s = ctx->stream;
+    if (vol->dtype == IVR_F16) {
+        if (skip) hipLaunchKernelGGL((k_runs<h16, true>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+        else hipLaunchKernelGGL((k_runs<h16, false>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+    } else {
+        if (skip) hipLaunchKernelGGL((k_runs<float, true>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+        else hipLaunchKernelGGL((k_runs<float, false>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+    }
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int ivr_launch_combine_runs(ivr_ctx *ctx, const RenderArgs &a, int runs)
+{
+    const size_t np = (size_t)a.W * a.H;
+    hipLaunchKernelGGL(k_combine_runs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, a.tile, runs,
+                       a.W, a.H, a.out8, a.out32);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }

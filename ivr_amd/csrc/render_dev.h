@@ -432,6 +432,24 @@ __device__ __forceinline__ void march_sort_last_rev(const RenderArgs &a, const f
     }
 }
 
+/* Run `run` (counted front to back) of one ray, composited from zero with
+ * early termination inside the run; nothing when the ray has fewer runs.
+ * The run's start is found by walking the boundaries before it. */
+template <typename VT, bool SKIP>
+__device__ __forceinline__ void march_run(const RenderArgs &a, const float4 *s_lut,
+                                          const unsigned long long *s_boff, const float *o, const float *d,
+                                          int run, Acc &acc, unsigned &cnt, unsigned &fetched)
+{
+    Ray r;
+    if (!ray_setup(a, o, d, r)) return;
+    int k = 0;
+    for (int t = 0; t < run && k < r.n; t++)
+        k = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
+    if (k >= r.n) return;
+    const int e = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
+    march_segment<VT, SKIP>(a, s_lut, s_boff, r, k, e, acc, cnt, fetched);
+}
+
 /* getlinearsample, non-BGQ (ray.c:122-178): reference weights and axes */
 template <typename VT>
 __device__ __forceinline__ float sample_ref(const RenderArgs &a, const unsigned long long *s_boff,
@@ -568,16 +586,15 @@ __device__ __forceinline__ void stage(const RenderArgs &a, float4 *s_lut, unsign
     __syncthreads();
 }
 
-/* Workgroup -> pixel block (2^blk_shift wide, IVR_FRAME_BLOCK pixels);
+/* Workgroup b -> pixel block (2^blk_shift wide, IVR_FRAME_BLOCK pixels);
  * returns false for padding.  Workgroups
  * are dealt to the 8 XCDs round robin (XCD = blockIdx % 8).  With a chunk
  * C > 0 each XCD renders runs of C consecutive blocks, so one L2 sees
  * neighbouring blocks, while the runs stay interleaved over the frame so
  * every XCD gets a share of the expensive centre.  Inside the block each
  * wavefront covers 2^s x 2^(6-s) pixels, s = wave_shift. */
-__device__ __forceinline__ bool block_pixel(const RenderArgs &a, int &i, int &j)
+__device__ __forceinline__ bool block_pixel(const RenderArgs &a, int b, int &i, int &j)
 {
-    const int b = blockIdx.x;
     int L = b;
     if (a.xcd_chunk > 0) {
         const int g = b >> 3, q = g / a.xcd_chunk, r = g - q * a.xcd_chunk;

@@ -809,7 +809,7 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     int i, j;
-    if (!block_pixel(a, i, j)) return; /* uniform per workgroup */
+    if (!block_pixel(a, blockIdx.x, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
     const bool inside = (i < a.H) && (j < a.W);
     float d[3] = {0.0f, 0.0f, 1.0f};

@@ -185,6 +185,19 @@ def test_sort_last_matches_oracle_tiles(ctx, composite, B):
         assert np.array_equal(s32.view(np.uint32), img32.view(np.uint32))
         assert sstats["samples"] == stats["samples"] and sstats["visited"] <= stats["visited"]
         print("sort-last with skipping: %d of %d samples fetched" % (sstats["visited"], stats["visited"]))
+        # the single-launch walk of all runs per ray (IVR_SL_SPLIT=0) gives the same image
+        os.environ["IVR_SL_SPLIT"] = "0"
+        try:
+            wctx = ivr.Context(0)
+        finally:
+            del os.environ["IVR_SL_SPLIT"]
+        wvol = wctx.volume(n, n, n, ivr.IVR_F32, brick=B)
+        wvol.upload(host)
+        for skip in (False, True):
+            w8, w32, wstats = _render(wctx, wvol, lut, ivr.config_desc(st, composite=mode, skip=skip), 96, 80)
+            assert np.array_equal(w32.view(np.uint32), img32.view(np.uint32)) and wstats["samples"] == stats["samples"]
+        wvol.close()
+        wctx.close()
     vol.close()
     ref32, s_ref = _oracle_sort_last(host, st, lut, B)
     m, rmse, same = _report("sort-last vs oracle tiles", img8, ref32, img32)
