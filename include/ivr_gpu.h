@@ -205,7 +205,7 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
 typedef struct ivr_batcher ivr_batcher;
 typedef struct ivr_request {
     unsigned long long id;  /* caller's request id                              */
-    int volume;             /* index from ivr_batcher_add_volume                */
+    int volume;             /* index from ivr_batcher_add_volume / _add_surrogate */
     ivr_frame_desc desc;    /* config-mode frame (state carries camera and size) */
     const float *lut;       /* RGBA LUT of lutsize <= 256 entries, copied at submit */
     int lutsize;
@@ -229,6 +229,13 @@ int ivr_batcher_create(ivr_ctx *ctx, unsigned long long hbm_budget, int max_slot
 int ivr_batcher_destroy(ivr_batcher *b);
 /* make a resident volume servable; returns its index (>= 0) or an error */
 int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol);
+/* make a trained surrogate servable: requests with the returned index are
+ * rendered from the network (one launch per request on the batch's stream,
+ * beside the batch kernel), bit-identical to ivr_surrogate_render.  The
+ * per-brick fallback flags are copied now; vol (kept alive by the caller,
+ * and charged to the budget only when it is registered itself) must be
+ * given when a flag is set.  No counterpart in the reference. */
+int ivr_batcher_add_surrogate(ivr_batcher *b, const ivr_surrogate *sg, const ivr_volume *vol);
 int ivr_batcher_submit(ivr_batcher *b, const ivr_request *req);
 /* retire finished batches, admit pending requests, launch a batch; with
  * block != 0 waits for the oldest batch when nothing else can proceed;

@@ -134,6 +134,7 @@ def _load():
         "ivr_batcher_create": (C.c_int, [P, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
         "ivr_batcher_destroy": (C.c_int, [P]),
         "ivr_batcher_add_volume": (C.c_int, [P, P]),
+        "ivr_batcher_add_surrogate": (C.c_int, [P, P, P]),
         "ivr_batcher_submit": (C.c_int, [P, C.POINTER(Request)]),
         "ivr_batcher_progress": (C.c_int, [P, C.c_int]),
         "ivr_batcher_poll": (C.c_int, [P, C.POINTER(Completion), C.c_int]),
@@ -496,6 +497,13 @@ class Batcher:
 
     def add_volume(self, vol):
         rc = lib.ivr_batcher_add_volume(self.h, vol.h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    def add_surrogate(self, sg, vol=None):
+        """serve a trained surrogate (vol: the plain volume of fallback bricks)"""
+        rc = lib.ivr_batcher_add_surrogate(self.h, sg.h, vol.h if vol is not None else None)
         if rc < 0:
             check(rc)
         return rc

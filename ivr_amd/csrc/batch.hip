@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "render_dev.h"
+#include "surrogate.h"
 
 namespace {
 
@@ -106,6 +107,15 @@ struct Slot {
     float4 *h_lut = nullptr;              /* pinned */
 };
 
+/* a servable resident: a plain volume, or a surrogate with its fallback
+ * flags (snapshot) and the optional plain volume of its fallback bricks */
+struct Resident {
+    const ivr_volume *vol = nullptr;
+    const ivr_surrogate *sg = nullptr;
+    unsigned char *d_fallback = nullptr;
+    size_t ncells() const { return sg ? sg->ncells : vol->ncells; }
+};
+
 struct Batch {
     bool active = false;
     int stream = 0;
@@ -124,7 +134,7 @@ struct ivr_batcher {
     unsigned long long base = 0; /* bytes of the resident volumes */
     int max_slots = 0, max_w = 0, max_h = 0;
     hipStream_t streams[2] = {};
-    std::vector<const ivr_volume *> vols;
+    std::vector<Resident> res;
     std::vector<Slot> slots;
     std::deque<Slot> pending; /* requests with their LUT copies */
     Batch batches[2];
@@ -197,6 +207,8 @@ int ivr_batcher_destroy(ivr_batcher *b)
         if (bt.h_prefix) (void)hipHostFree(bt.h_prefix);
     }
     for (auto &s : b->streams) (void)hipStreamDestroy(s);
+    for (Resident &r : b->res)
+        if (r.sg) ivr_dfree(b->ctx, r.d_fallback, r.sg->nbricks);
     delete b;
     return IVR_OK;
 }
@@ -210,9 +222,38 @@ int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol)
     if (vol->nbricks > IVR_MAX_BRICKS / 8) return ivr_fail(IVR_ERR_UNSUPPORTED, "too many bricks to batch");
     b->used += bytes;
     b->base += bytes;
-    b->vols.push_back(vol);
+    Resident r;
+    r.vol = vol;
+    b->res.push_back(r);
     b->st.bytes_in_use = b->used;
-    return (int)b->vols.size() - 1;
+    return (int)b->res.size() - 1;
+}
+
+int ivr_batcher_add_surrogate(ivr_batcher *b, const ivr_surrogate *sg, const ivr_volume *vol)
+{
+    if (!b || !sg) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (vol && (vol->nbricks != sg->nbricks || vol->B != sg->B))
+        return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    std::vector<unsigned char> h(sg->nbricks);
+    IVR_HIP(hipMemcpy(h.data(), sg->d_fallback, sg->nbricks, hipMemcpyDeviceToHost));
+    for (unsigned char c : h)
+        if (c && !vol) return ivr_fail(IVR_ERR_ARG, "a brick falls back to the plain volume, which was not given");
+    /* the network's resident bytes: hash tables, parameters, operand fragments, cell ranges */
+    const unsigned long long bytes = sg->ntable * 4ull + SG_NPARAM * 4ull + SG_NFRAG * 1024ull +
+                                     sg->ncells * 9ull + sg->nbricks;
+    if (b->used + bytes > b->budget)
+        return ivr_fail(IVR_ERR_BUDGET, "surrogate of %llu bytes exceeds the HBM budget", bytes);
+    Resident r;
+    r.vol = vol;
+    r.sg = sg;
+    int rc = ivr_dalloc(b->ctx, (void **)&r.d_fallback, sg->nbricks);
+    if (rc) return rc;
+    IVR_HIP(hipMemcpy(r.d_fallback, h.data(), sg->nbricks, hipMemcpyHostToDevice));
+    b->used += bytes;
+    b->base += bytes;
+    b->res.push_back(r);
+    b->st.bytes_in_use = b->used;
+    return (int)b->res.size() - 1;
 }
 
 namespace {
@@ -230,9 +271,8 @@ void reject(ivr_batcher *b, const ivr_request &r, int code)
 
 unsigned long long charge_of(const ivr_batcher *b, const ivr_request &r)
 {
-    const ivr_volume *v = b->vols[r.volume];
     return (unsigned long long)r.desc.state.winwidth * r.desc.state.winheight * 4ull + BATCH_LUT * 16ull +
-           sizeof(RenderArgs) + (r.desc.skip ? v->ncells : 0ull);
+           sizeof(RenderArgs) + (r.desc.skip ? b->res[r.volume].ncells() : 0ull);
 }
 
 int retire(ivr_batcher *b, Batch &bt, bool wait)
@@ -285,13 +325,13 @@ int admit(ivr_batcher *b)
         }
         if (b->used + ch > b->budget) break; /* FIFO: wait for memory to free up */
         Slot &s = b->slots[free_slot];
-        const ivr_volume *v = b->vols[p.req.volume];
-        if (p.req.desc.skip && s.cells_cap < v->ncells) {
+        const size_t nc = b->res[p.req.volume].ncells();
+        if (p.req.desc.skip && s.cells_cap < nc) {
             ivr_dfree(b->ctx, s.d_cells, s.cells_cap);
             s.cells_cap = 0;
-            int rc = ivr_dalloc(b->ctx, (void **)&s.d_cells, v->ncells);
+            int rc = ivr_dalloc(b->ctx, (void **)&s.d_cells, nc);
             if (rc) return rc;
-            s.cells_cap = v->ncells;
+            s.cells_cap = nc;
         }
         s.req = p.req;
         s.lut.swap(p.lut);
@@ -324,11 +364,12 @@ int launch(ivr_batcher *b)
     /* FIFO order of admission */
     std::sort(ready.begin(), ready.end(), [&](int x, int y) { return b->slots[x].t_admit < b->slots[y].t_admit; });
     hipStream_t st = b->streams[bt->stream];
-    int total = 0;
+    int total = 0, nplain = 0;
     bt->slots = ready;
     for (size_t q = 0; q < ready.size(); q++) {
         Slot &s = b->slots[ready[q]];
-        const ivr_volume *v = b->vols[s.req.volume];
+        const Resident &res = b->res[s.req.volume];
+        const ivr_volume *v = res.vol;
         const ivr_frame_desc &d = s.req.desc;
         memcpy(s.h_lut, s.lut.data(), s.lut.size() * sizeof(float));
         IVR_HIP(hipMemcpyAsync(s.d_lut, s.h_lut, s.lut.size() * sizeof(float), hipMemcpyHostToDevice, st));
@@ -341,9 +382,18 @@ int launch(ivr_batcher *b)
         fr.d_rgba8 = s.d_img;
         fr.d_rgba32 = nullptr;
         fr.d_counters = s.d_ctr;
+        if (res.sg) { /* its own launch, ahead of the batch kernel on this stream */
+            IVR_HIP(hipMemsetAsync(s.d_ctr, 0, 16, st));
+            int rc = ivr_surrogate_launch_on(b->ctx, st, res.sg, v, &tf, &d, &fr, d.skip ? s.d_cells : nullptr,
+                                             res.d_fallback);
+            if (rc) return rc;
+            s.launched = true;
+            continue;
+        }
+        const int q2 = nplain++; /* plain requests fill the batch's argument table in order */
         ivr_cam cam;
         ivr_cam_setup(&d.state, v->n[0], v->n[1], v->n[2], &cam);
-        RenderArgs &a = bt->h_args[q];
+        RenderArgs &a = bt->h_args[q2];
         ivr_fill_args(a, cam, v, &tf, &d, &fr);
         a.nblk_x = (fr.W + 15) / 16;
         a.cellempty = d.skip ? s.d_cells : nullptr;
@@ -352,18 +402,20 @@ int launch(ivr_batcher *b)
             if (rc) return rc;
         }
         IVR_HIP(hipMemsetAsync(s.d_ctr, 0, 16, st));
-        bt->h_prefix[q] = total;
+        bt->h_prefix[q2] = total;
         total += a.nblk_x * ((fr.H + 15) / 16);
         s.launched = true;
     }
-    bt->h_prefix[ready.size()] = total;
-    IVR_HIP(hipMemcpyAsync(bt->d_args, bt->h_args, sizeof(RenderArgs) * ready.size(), hipMemcpyHostToDevice, st));
-    IVR_HIP(hipMemcpyAsync(bt->d_prefix, bt->h_prefix, sizeof(int) * (ready.size() + 1), hipMemcpyHostToDevice, st));
-    IVR_HIP(hipMemsetAsync(bt->d_counter, 0, sizeof(unsigned), st));
-    const int grid = std::min(total, 2048);
-    hipLaunchKernelGGL(k_render_batch, dim3(grid), dim3(IVR_BLOCK), 0, st, bt->d_args, bt->d_prefix,
-                       (int)ready.size(), total, bt->d_counter);
-    IVR_HIP(hipGetLastError());
+    if (nplain > 0) {
+        bt->h_prefix[nplain] = total;
+        IVR_HIP(hipMemcpyAsync(bt->d_args, bt->h_args, sizeof(RenderArgs) * nplain, hipMemcpyHostToDevice, st));
+        IVR_HIP(hipMemcpyAsync(bt->d_prefix, bt->h_prefix, sizeof(int) * (nplain + 1), hipMemcpyHostToDevice, st));
+        IVR_HIP(hipMemsetAsync(bt->d_counter, 0, sizeof(unsigned), st));
+        const int grid = std::min(total, 2048);
+        hipLaunchKernelGGL(k_render_batch, dim3(grid), dim3(IVR_BLOCK), 0, st, bt->d_args, bt->d_prefix, nplain,
+                           total, bt->d_counter);
+        IVR_HIP(hipGetLastError());
+    }
     for (int si : ready) {
         Slot &s = b->slots[si];
         const size_t bytes = (size_t)s.req.desc.state.winwidth * s.req.desc.state.winheight * 4;
@@ -385,7 +437,7 @@ int ivr_batcher_submit(ivr_batcher *b, const ivr_request *r)
     b->st.submitted++;
     const ivr_frame_desc &d = r->desc;
     int code = 0;
-    if (r->volume < 0 || r->volume >= (int)b->vols.size()) code = IVR_ERR_ARG;
+    if (r->volume < 0 || r->volume >= (int)b->res.size()) code = IVR_ERR_ARG;
     else if (d.mode != IVR_MODE_CONFIG || d.composite != IVR_FUSED || !(d.step_vox > 0.0f)) code = IVR_ERR_UNSUPPORTED;
     else if (d.state.winwidth < 1 || d.state.winheight < 1 || d.state.winwidth > b->max_w ||
              d.state.winheight > b->max_h)

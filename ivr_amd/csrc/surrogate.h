@@ -51,4 +51,11 @@ struct ivr_surrogate {
     unsigned char *d_cellempty = nullptr;
 };
 
+/* a surrogate frame on stream st with the caller's macrocell buffer (ncells
+ * bytes, used with skipping) and per-brick fallback flags (device, nbricks
+ * bytes; vol must be given when one is set); fr's counters are not cleared */
+int ivr_surrogate_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                            const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                            const unsigned char *fallback);
+
 #endif

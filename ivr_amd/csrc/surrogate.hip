@@ -917,8 +917,12 @@ __global__ void k_sqerr(const uchar4 *x, const uchar4 *y, size_t n, unsigned lon
     count_add(out, v);
 }
 
-int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
-              const ivr_frame_desc *d, ivr_frame *fr, const unsigned char *fallback, int only_brick)
+/* the surrogate frame's kernels on stream st: macrocell codes of the frame's
+ * LUT into `cells` (with skipping), then k_render_sg; fr's counters must be
+ * cleared by the caller */
+int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                 const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                 const unsigned char *fallback, int only_brick)
 {
     if (d->mode != IVR_MODE_CONFIG) return ivr_fail(IVR_ERR_UNSUPPORTED, "surrogate renders config mode only");
     if (d->state.winwidth != fr->W || d->state.winheight != fr->H)
@@ -953,28 +957,47 @@ int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const
     S.fallback = fallback;
     S.only_brick = only_brick;
     for (int r = 0; r < 3; r++) S.a.nc[r] = s->nc[r];
-    S.a.cellempty = s->d_cellempty;
+    S.a.cellempty = cells;
     ivr_set_grid(S.a, ctx, fr->W, fr->H);
+    const bool skip = d->skip && s->ncells > 0;
+    if (skip) {
+        ivr_volume geo_cells; /* the surrogate's own macrocell ranges */
+        geo_cells.ncells = s->ncells;
+        for (int r = 0; r < 3; r++) geo_cells.nc[r] = s->nc[r];
+        geo_cells.d_cellrange = s->d_cellrange;
+        int rc = ivr_launch_cells_on(st, &geo_cells, S.a, cells);
+        if (rc) return rc;
+    }
+    const size_t lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
+    if (skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, st, S);
+    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, st, S);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+              const ivr_frame_desc *d, ivr_frame *fr, const unsigned char *fallback, int only_brick)
+{
     IVR_HIP(hipMemsetAsync(fr->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
     fr->timed = 1;
     fr->composited = 0;
     IVR_HIP(hipEventRecord(fr->e[0], ctx->stream));
-    const bool skip = d->skip && s->ncells > 0;
-    if (skip) {
-        ivr_volume cells; /* the surrogate's own macrocell ranges */
-        cells.ncells = s->ncells;
-        for (int r = 0; r < 3; r++) cells.nc[r] = s->nc[r];
-        cells.d_cellrange = s->d_cellrange;
-        int rc = ivr_launch_cells_on(ctx->stream, &cells, S.a, s->d_cellempty);
-        if (rc) return rc;
-    }
-    const size_t lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
-    if (skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, ctx->stream, S);
-    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, ctx->stream, S);
-    IVR_HIP(hipGetLastError());
+    int rc = sg_launch_on(ctx, ctx->stream, s, vol, tf, d, fr, s->d_cellempty, fallback, only_brick);
+    if (rc) return rc;
     IVR_HIP(hipEventRecord(fr->e[1], ctx->stream));
     return IVR_OK;
 }
+
+} // namespace
+
+int ivr_surrogate_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                            const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                            const unsigned char *fallback)
+{
+    return sg_launch_on(ctx, st, s, vol, tf, d, fr, cells, fallback, -1);
+}
+
+namespace {
 
 } // namespace
 

@@ -94,3 +94,52 @@ def test_poisson_driver_reports_latency(ctx, vols):
     print({k: v for k, v in res.items() if k != "records"})
     assert res["ok"] == 40 and res["p99_ms"] >= res["p50_ms"] > 0
     bt.close()
+
+
+def test_surrogate_requests_in_the_mix(ctx, vols):
+    """requests served from a trained surrogate (with and without fallback
+    bricks, skipping on and off) next to plain requests: each image and its
+    sample count equal ivr_surrogate_render of the same request alone"""
+    vol = vols[0]
+    sg = ctx.surrogate(vol, log2_table=12)
+    sg.train(vol, 20, 1 << 16)
+    flags = np.zeros(sg.nbricks, np.uint8)
+    flags[::3] = 1  # every third brick renders from the plain volume
+    bt = ctx.batcher(1 << 34, 4, 256, 256)
+    assert bt.add_volume(vol) == 0
+    sg.set_fallback(flags)
+    with pytest.raises(ivr.IvrError):
+        bt.add_surrogate(sg)  # fallback bricks need the plain volume
+    assert bt.add_surrogate(sg, vol) == 1
+    sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
+    assert bt.add_surrogate(sg) == 2  # no fallback: the network alone
+    sizes = [(64, 48), (128, 96), (96, 96)]
+    reqs = serving.make_requests(12, 1, seed=21, tf_seed=2005, resolutions=sizes)
+    for k, r in enumerate(reqs):
+        r["volume"] = k % 3
+        r["desc"].skip = k % 2 == 1
+        bt.submit(r["id"], r["volume"], r["desc"], r["lut"])
+    done = {}
+    while len(done) < len(reqs):
+        bt.progress(True)
+        for c in bt.poll():
+            done[c["id"]] = c
+    for r in reqs:
+        W, H = r["desc"].state.winwidth, r["desc"].state.winheight
+        tf = ctx.tf(r["lut"])
+        fr = ctx.frame(W, H)
+        if r["volume"] == 0:
+            ctx.render(vol, tf, r["desc"], fr)
+        else:
+            sg.set_fallback(flags if r["volume"] == 1 else np.zeros(sg.nbricks, np.uint8))
+            sg.render(vol if r["volume"] == 1 else None, tf, r["desc"], fr)
+        alone = fr.download()
+        s_alone = fr.stats()["samples"]
+        fr.close()
+        tf.close()
+        c = done[r["id"]]
+        assert c["status"] == 0
+        assert np.array_equal(c["image"], alone), (r["id"], r["volume"])
+        assert c["samples"] == s_alone
+    bt.close()
+    sg.close()
