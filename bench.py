@@ -57,7 +57,11 @@ def main():
                     help="brick compositing (default per config)")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--train-seconds", type=float, default=60.0, help="--config 4: training time budget")
+    ap.add_argument("--train-seconds", type=float, default=60.0, help="--config 4/5: training time budget")
+    ap.add_argument("--sg-volume", type=int, default=2, choices=[2, 3],
+                    help="--config 4: train on the C2 (512^3 fp32) or the C3 (1024^3 fp16) volume")
+    ap.add_argument("--serve-surrogate", action="store_true",
+                    help="--config 5: also serve the 1024^3 volume from a surrogate trained for --train-seconds")
     ap.add_argument("--requests", type=int, default=400, help="--config 5: requests per arrival rate")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (for scaling PMC byte counters)")
@@ -224,30 +228,11 @@ def _frame_ms(ctx, fn, steps):
     return ctx.elapsed(0, 1) / steps
 
 
-def bench_surrogate(args):
-    """C4: per-brick hash grids + BF16 MFMA MLP trained on the C2 volume
-    with 2^24 samples per Adam step for a time budget, then 1920x1080 frames
-    from the surrogate against the plain render: PSNR/SSIM, frames/s, and
-    the per-brick fallback below 40 dB."""
-    import numpy as np
-    import ivr_amd as ivr
-    from ivr_amd import configs
-    cfg = configs.CONFIGS[2]
-    ctx = ivr.Context(0)
-    vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
-    vol.generate(configs.gen_params(2))
-    W, H = 1920, 1080
-    lut = configs.tf_lut(2)
-    tf = ctx.tf(lut)
-    desc = ivr.config_desc(ivr.orbit_state(3004, W, H))  # camera seed of C4
-    fr = ctx.frame(W, H)
-    plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), args.steps)
-    plain = fr.download()
+def train_surrogate(ctx, vol, seconds, batch=1 << 24, chunk=5):
+    """Adam steps of `batch` samples in chunks until `seconds` of training"""
     sg = ctx.surrogate(vol)
-    batch = 1 << 24
     losses, steps, t_train = [], 0, 0.0
-    chunk = 5
-    while t_train < args.train_seconds:
+    while t_train < seconds:
         ctx.sync()
         t = time.perf_counter()
         losses.append(sg.train(vol, chunk, batch))
@@ -255,6 +240,32 @@ def bench_surrogate(args):
         t_train += time.perf_counter() - t
         steps += chunk
         log("surrogate: %d steps, %.1fs, loss %.3e" % (steps, t_train, losses[-1]))
+    return sg, losses, steps, t_train
+
+
+def bench_surrogate(args):
+    """C4: per-brick hash grids + BF16 MFMA MLP trained on the C2 volume
+    (or, with --sg-volume 3, the 1024^3 fp16 C3 volume) with 2^24 samples
+    per Adam step for a time budget, then 1920x1080 frames from the
+    surrogate against the plain render: PSNR/SSIM, frames/s, and the
+    per-brick fallback below 40 dB."""
+    import numpy as np
+    import ivr_amd as ivr
+    from ivr_amd import configs
+    c = args.sg_volume
+    cfg = configs.CONFIGS[c]
+    ctx = ivr.Context(0)
+    vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
+    vol.generate(configs.gen_params(c))
+    W, H = 1920, 1080
+    lut = configs.tf_lut(c)
+    tf = ctx.tf(lut)
+    desc = ivr.config_desc(ivr.orbit_state(3004, W, H))  # camera seed of C4
+    fr = ctx.frame(W, H)
+    plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), args.steps)
+    plain = fr.download()
+    batch = 1 << 24
+    sg, losses, steps, t_train = train_surrogate(ctx, vol, args.train_seconds, batch)
     step_ms = t_train * 1e3 / steps
     sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
     ns_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
@@ -279,9 +290,11 @@ def bench_surrogate(args):
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "bf16 MLP on MFMA, fp32 tables and accumulation",
-        "data": "synthetic C2 volume (512^3 fp32), surrogate trained on the GPU",
+        "data": "synthetic %s volume (%s), surrogate trained on the GPU"
+                % ("C2" if c == 2 else "C3", "512^3 fp32" if c == 2 else "1024^3 fp16"),
         "config": {
-            "workload": "C4 surrogate of C2: 8 bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1",
+            "workload": "C4 surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1"
+                        % ("C2" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks),
             "image": "%dx%d" % (W, H),
             "train_steps": steps, "train_samples_per_step": batch, "train_ms_per_step": round(step_ms, 2),
             "train_mlp_tflops": round(flops_step / (step_ms * 1e-3) / 1e12, 2),
@@ -312,24 +325,57 @@ def bench_serving(args):
     jittered TF and a resolution from {256^2, 512^2, 1024^2, 1920x1080}."""
     import ivr_amd as ivr
     from ivr_amd import configs, serving
+    import numpy as np
     ctx = ivr.Context(0)
     vols = []
     for (n, dt, c, brick) in ((256, ivr.IVR_F32, 2, 256), (512, ivr.IVR_F32, 2, 256), (1024, ivr.IVR_F16, 3, 256)):
         v = ctx.volume(n, n, n, dt, brick=brick)
         v.generate(configs.gen_params(c))
         vols.append(v)
+
+    def sweep(bt, tag):
+        rows = []
+        for k, rate in enumerate((50.0, 100.0, 200.0, 400.0, 1e9)):
+            reqs = serving.make_requests(args.requests, len(vols), seed=4100 + k, tf_seed=2005)
+            res = serving.run_poisson(bt, reqs, rate, seed=4000 + k)
+            row = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in res.items() if kk != "records"}
+            row["rate"] = "burst" if rate >= 1e9 else rate
+            # latency of the requests of the 1024^3 resident (volume 2) apart
+            lat3 = [(r["t_done"] - r["t_arrival"]) * 1e3 for r in res["records"].values()
+                    if r["status"] == 0 and reqs[r["id"] - 1]["volume"] == 2]
+            if lat3:
+                row["vol1024_p50_ms"] = round(float(np.percentile(lat3, 50)), 3)
+                row["vol1024_p99_ms"] = round(float(np.percentile(lat3, 99)), 3)
+            log("serving (%s): %s" % (tag, row))
+            rows.append(row)
+        return rows
+
     bt = ctx.batcher(64 << 30, 16, 1920, 1080)
     for v in vols:
         bt.add_volume(v)
-    rows = []
-    for k, rate in enumerate((50.0, 100.0, 200.0, 400.0, 1e9)):
-        reqs = serving.make_requests(args.requests, len(vols), seed=4100 + k, tf_seed=2005)
-        res = serving.run_poisson(bt, reqs, rate, seed=4000 + k)
-        row = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in res.items() if kk != "records"}
-        row["rate"] = "burst" if rate >= 1e9 else rate
-        log("serving: %s" % row)
-        rows.append(row)
+    rows = sweep(bt, "plain")
     st = bt.stats()
+    bt.close()
+    sg_part = None
+    if args.serve_surrogate:
+        # the 1024^3 resident served from its surrogate (fallback bricks from the plain volume)
+        sg, losses, steps, t_train = train_surrogate(ctx, vols[2], args.train_seconds)
+        lut = configs.tf_lut(3)
+        tf = ctx.tf(lut)
+        desc = ivr.config_desc(ivr.orbit_state(3004, 1920, 1080), skip=True)
+        bpsnr, nfall = sg.evaluate(vols[2], tf, desc, threshold_db=40.0)
+        tf.close()
+        bt = ctx.batcher(64 << 30, 16, 1920, 1080)
+        bt.add_volume(vols[0])
+        bt.add_volume(vols[1])
+        bt.add_surrogate(sg, vols[2])
+        srows = sweep(bt, "surrogate")
+        bt.close()
+        sg_part = {"train_steps": steps, "train_s": round(t_train, 1), "loss_last": losses[-1],
+                   "fallback_bricks": int(nfall), "bricks": sg.nbricks,
+                   "brick_psnr_db_min": round(float(bpsnr.min()), 2), "rates": srows,
+                   "capacity_frames_per_s": srows[-1]["frames_per_s"]}
+        sg.close()
     out = {
         "metric": "steering service p99 latency (arrival -> RGBA8 on host) and frames/s",
         "value": rows[-2]["p99_ms"],
@@ -340,10 +386,10 @@ def bench_serving(args):
         "config": {"workload": "C5: 256^3 fp32, 512^3 fp32, 1024^3 fp16 resident; 16 slots; 64 GiB budget",
                    "requests_per_rate": args.requests, "rates": rows,
                    "capacity_frames_per_s": rows[-1]["frames_per_s"],
-                   "batcher": {k: v for k, v in st.items()} if isinstance(st, dict) else None},
+                   "batcher": {k: v for k, v in st.items()} if isinstance(st, dict) else None,
+                   "surrogate_1024": sg_part},
     }
     print(json.dumps(out), flush=True)
-    bt.close()
     for v in vols:
         v.close()
     ctx.close()

@@ -47,6 +47,8 @@ def run_poisson(batcher, reqs, rate, seed):
             done[c["id"]] = c
         if k < len(reqs) and not batcher.busy():
             time.sleep(max(0.0, min(arrivals[k] - ivr.now(), 0.002)))
+    for c in done.values():
+        c["t_arrival"] = float(arrivals[c["id"] - 1])
     t_end = max(c["t_done"] for c in done.values())
     lat = np.array([(c["t_done"] - arrivals[c["id"] - 1]) * 1e3 for c in done.values() if c["status"] == 0])
     ok = int((np.array([c["status"] for c in done.values()]) == 0).sum())
