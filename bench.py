@@ -522,6 +522,14 @@ def reference_cpu(ctx, vol, st, lut, W, H):
     ref = oracle.ref_lib()
     if ref is None:
         return None
+    if max(vol.info()["dims"]) > 512:
+        # getsamplesonray's last sample may pass the exit face by half a
+        # 0.001-step (ray.c:253-261), more than a voxel beyond 512 per axis, and
+        # getlinearsample has no lower index clamp (ray.c:126-135): an exit
+        # through a low face then reads a slice before the volume buffer
+        log("reference ray.c: not run on a %s volume (it reads before its buffer there)"
+            % "x".join(str(d) for d in vol.info()["dims"]))
+        return None
     host = vol.download() * np.float32(255.0)
     nz, ny, nx = host.shape
     gv = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=vol.info()["brick"])

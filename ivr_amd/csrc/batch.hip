@@ -238,8 +238,9 @@ int ivr_batcher_add_surrogate(ivr_batcher *b, const ivr_surrogate *sg, const ivr
     IVR_HIP(hipMemcpy(h.data(), sg->d_fallback, sg->nbricks, hipMemcpyDeviceToHost));
     for (unsigned char c : h)
         if (c && !vol) return ivr_fail(IVR_ERR_ARG, "a brick falls back to the plain volume, which was not given");
-    /* the network's resident bytes: hash tables, parameters, operand fragments, cell ranges */
-    const unsigned long long bytes = sg->ntable * 4ull + SG_NPARAM * 4ull + SG_NFRAG * 1024ull +
+    /* the network's resident bytes: hash tables (fp32 and the half copy it renders from),
+     * parameters, operand fragments, cell ranges */
+    const unsigned long long bytes = sg->ntable * 6ull + SG_NPARAM * 4ull + SG_NFRAG * 1024ull +
                                      sg->ncells * 9ull + sg->nbricks;
     if (b->used + bytes > b->budget)
         return ivr_fail(IVR_ERR_BUDGET, "surrogate of %llu bytes exceeds the HBM budget", bytes);

@@ -37,6 +37,7 @@ struct ivr_surrogate {
     unsigned dense_n1[SG_L] = {};
     size_t ntable = 0;
     float *tables = nullptr, *gtables = nullptr, *mt = nullptr, *vt = nullptr;
+    void *tables16 = nullptr; /* half-precision copy of tables, read by rendering */
     float *params = nullptr, *gparams = nullptr, *mp = nullptr, *vp = nullptr;
     void *frag = nullptr;
     float *d_loss = nullptr;

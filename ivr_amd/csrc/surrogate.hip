@@ -48,7 +48,8 @@ __host__ __device__ __forceinline__ int kperm(int ks, int g, int j) { return (2 
 /* ---------------------------------------------------------------- encoding */
 
 struct Enc {
-    const float *table; /* [nbricks][L][T][F] */
+    const float *table;    /* [nbricks][L][T][F] */
+    const __half2 *table16; /* the same in half precision (rendering), or null */
     unsigned T;
     int res[SG_L];
     unsigned n1[SG_L];  /* N+1 for dense levels, 0 for hashed */
@@ -75,18 +76,26 @@ __device__ __forceinline__ void level_cell(const Enc &e, int l, const float *x, 
     }
 }
 
+__device__ __forceinline__ float2 table_at(const float2 *tb, unsigned i) { return tb[i]; }
+__device__ __forceinline__ float2 table_at(const __half2 *tb, unsigned i) { return __half22float2(tb[i]); }
+
+/* level l of the encoding at local position x: trilinear blend of the 8
+ * corner features, from the fp32 tables (TT = float2, training) or their
+ * half-precision copy (TT = __half2, rendering: half the bytes per corner) */
+template <typename TT = float2>
 __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, int l, const float *x)
 {
     int v0[3];
     float w[3];
     level_cell(e, l, x, v0, w);
-    const float2 *tb = (const float2 *)(e.table + ((size_t)brick * SG_L + l) * (size_t)e.T * SG_F);
+    const size_t base = ((size_t)brick * SG_L + l) * (size_t)e.T;
+    const TT *tb = sizeof(TT) == sizeof(float2) ? (const TT *)e.table + base : (const TT *)e.table16 + base;
     float2 f = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
         const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const float2 t = tb[corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz)];
+        const float2 t = table_at(tb, corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz));
         f.x = fmaf(wc, t.x, f.x);
         f.y = fmaf(wc, t.y, f.y);
     }
@@ -129,6 +138,7 @@ __device__ __forceinline__ void gather_samples(const float *x, int b, SampleSet 
     }
 }
 
+template <typename TT = float2>
 __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, bf16x8 E[4])
 {
     const int g = (threadIdx.x & 63) >> 4;
@@ -136,7 +146,7 @@ __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, b
     for (int nt = 0; nt < 4; nt++) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float2 f = encode_level(e, s.b[nt], 4 * g + q, s.x[nt]);
+            const float2 f = encode_level<TT>(e, s.b[nt], 4 * g + q, s.x[nt]);
             E[nt][2 * q] = (__bf16)f.x;
             E[nt][2 * q + 1] = (__bf16)f.y;
         }
@@ -497,6 +507,13 @@ __global__ void k_init(float *p, size_t n, float scale, unsigned seed)
     p[i] = ((float)(h >> 8) * (1.0f / 16777216.0f) * 2.0f - 1.0f) * scale;
 }
 
+/* fp32 feature pairs -> half2 (round to nearest even) */
+__global__ __launch_bounds__(256) void k_half(const float2 *src, __half2 *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = __float22half2_rn(src[i]);
+}
+
 } // namespace
 
 /* ================================================================= host API */
@@ -507,6 +524,7 @@ Enc make_enc(const ivr_surrogate *s)
 {
     Enc e;
     e.table = s->tables;
+    e.table16 = (const __half2 *)s->tables16;
     e.T = s->T;
     for (int l = 0; l < SG_L; l++) {
         e.res[l] = s->res[l];
@@ -532,9 +550,14 @@ void volume_args(RenderArgs &a, const ivr_volume *v)
     a.dtype_f16 = v->dtype == IVR_F16;
 }
 
+/* after every change of the weights: the MLP's bf16 operand fragments and
+ * the half-precision copy of the hash tables that rendering reads */
 int refresh_frags(ivr_ctx *ctx, ivr_surrogate *s)
 {
     hipLaunchKernelGGL(k_frags, dim3(SG_NFRAG), dim3(64), 0, ctx->stream, s->params, (bf16x8 *)s->frag);
+    const size_t n2 = s->ntable / 2;
+    hipLaunchKernelGGL(k_half, dim3((unsigned)std::min<size_t>((n2 + 255) / 256, 65536)), dim3(256), 0, ctx->stream,
+                       (const float2 *)s->tables, (__half2 *)s->tables16, n2);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }
@@ -564,6 +587,7 @@ int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogat
     s->ncells = vol->ncells;
     int rc;
     if ((rc = ivr_dalloc(ctx, (void **)&s->tables, s->ntable * 4)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->tables16, s->ntable * 2)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->gtables, s->ntable * 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->mt, s->ntable * 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->vt, s->ntable * 4)) ||
@@ -612,6 +636,7 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
     if (ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         ivr_dfree(ctx, s->tables, s->ntable * 4);
+        ivr_dfree(ctx, s->tables16, s->ntable * 2);
         ivr_dfree(ctx, s->gtables, s->ntable * 4);
         ivr_dfree(ctx, s->mt, s->ntable * 4);
         ivr_dfree(ctx, s->vt, s->ntable * 4);
@@ -868,7 +893,7 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
         SampleSet ss;
         gather_samples(x, b, ss);
         bf16x8 E[4];
-        encode_frags(S.enc, ss, E);
+        encode_frags<__half2>(S.enc, ss, E);
         Fwd f;
         mlp_forward(S.mlp, E, f);
         const float yv = own(f.y);
