@@ -19,9 +19,9 @@ CONFIGS = {
             dims=(512, 512, 512), dtype=ivr.IVR_F32, brick=256, image=(1024, 1024), skip=False,
             composite=ivr.SORT_LAST,
             gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.4)),
-    3: dict(name="C3 1024^3 fp16 Perlin+blobs (70% zero), 4x4x4 bricks of 256^3 -> 1920x1080",
+    3: dict(name="C3 1024^3 fp16 Perlin+blobs (70% zero), 4x4x4 bricks of 256^3, sort-last -> 1920x1080",
             dims=(1024, 1024, 1024), dtype=ivr.IVR_F16, brick=256, image=(1920, 1080), skip=True,
-            composite=ivr.FUSED, gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.7)),
+            composite=ivr.SORT_LAST, gen=dict(kind=ivr.GEN_NOISE_BLOBS, nblobs=32, empty_frac=0.7)),
 }
 
 
