@@ -256,6 +256,30 @@ def test_hierarchical_skip_on_sparse_volume(ctx, B):
     vol.close()
 
 
+@pytest.mark.parametrize("composite", ["SORT_LAST", "SORT_LAST_TILES"])
+def test_fp16_sort_last_with_skipping_matches_oracle(ctx, composite):
+    """an fp16 volume of 4x3x5 bricks (the C3 storage, at test size), sort-last
+    with macrocell skipping on and off, bit for bit against the oracle's
+    per-brick tiles of the fp16-rounded field"""
+    gp = configs.gen_params(3)
+    nx, ny, nz = 96, 80, 72
+    host = ivr.generate_host(gp, nx, ny, nz)
+    h16 = host.astype(np.float16).astype(np.float32)
+    lut = configs.tf_lut(3)
+    W, H = 112, 96
+    st = configs.state(3, W, H)
+    vol = ctx.volume(nx, ny, nz, ivr.IVR_F16, brick=16 if composite == "SORT_LAST_TILES" else 32)
+    vol.upload(host)
+    B = vol.info()["brick"]
+    ref32, s_ref = _oracle_sort_last(h16, st, lut, B)
+    mode = getattr(ivr, composite)
+    for skip in (False, True):
+        img8, img32, s = _render(ctx, vol, lut, ivr.config_desc(st, composite=mode, skip=skip), W, H)
+        _report("fp16 %s B=%d skip=%d" % (composite, B, skip), img8, ref32, img32)
+        assert np.array_equal(img32.view(np.uint32), ref32.view(np.uint32)) and s["samples"] == s_ref
+    vol.close()
+
+
 @pytest.mark.parametrize("B", [16, 32])
 def test_ragged_sizes_match_oracle(ctx, B):
     """a 53x37x29 volume (no axis a multiple of the brick; bricks partly
