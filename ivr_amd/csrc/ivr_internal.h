@@ -114,6 +114,7 @@ struct RenderArgs {
     int bshift, bmask, P;
     int nbricks;
     const void *vox_data;
+    unsigned vox_bytes; /* bytes of vox_data when below 4 GiB - 1 MiB (buffer loads), else 0 */
     int dtype_f16;
     const unsigned long long *boff;
     const unsigned char *cellempty;

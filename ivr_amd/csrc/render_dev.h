@@ -219,6 +219,76 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs &a, const float *o, c
     return true;
 }
 
+/* The 8-voxel footprint of fetch8 through a raw buffer resource over the
+ * volume: byte offset o of (j,i,k), cy and cz the byte strides of a row and
+ * a slice.  Offsets at or past the resource's size return zeros without a
+ * memory access, which lets a lane skip a footprint without a branch. */
+template <typename VT>
+__device__ __forceinline__ void fetch8_buf(__amdgpu_buffer_rsrc_t rs, unsigned o, unsigned cy, unsigned cz,
+                                           float v[8])
+{
+    if (sizeof(VT) == 4) {
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        const f2 a0 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0));
+        const f2 a1 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cy, 0, 0));
+        const f2 a2 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cz, 0, 0));
+        const f2 a3 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cy + cz, 0, 0));
+        v[0] = a0.x; v[1] = a0.y; v[2] = a1.x; v[3] = a1.y;
+        v[4] = a2.x; v[5] = a2.y; v[6] = a3.x; v[7] = a3.y;
+    } else {
+        typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+        const h2 a0 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0));
+        const h2 a1 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cy, 0, 0));
+        const h2 a2 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cz, 0, 0));
+        const h2 a3 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cy + cz, 0, 0));
+        v[0] = (float)a0.x; v[1] = (float)a0.y; v[2] = (float)a1.x; v[3] = (float)a1.y;
+        v[4] = (float)a2.x; v[5] = (float)a2.y; v[6] = (float)a3.x; v[7] = (float)a3.y;
+    }
+}
+
+/* march_segment without macrocell skipping, software-pipelined: the next
+ * sample's footprint is requested before the current sample is composited,
+ * and a lane whose next sample shares its floor index requests one past the
+ * end of the buffer (zeros, no memory traffic) and keeps its voxels, so the
+ * reuse is kept with no branch around the loads (a branch makes the
+ * compiler wait for every load at its join).  Same float operations and
+ * counts as march_segment. */
+template <typename VT>
+__device__ __forceinline__ void march_segment_pipe(const RenderArgs &a, const float4 *s_lut,
+                                                   const unsigned long long *s_boff, const Ray &r, int k0,
+                                                   int k1, Acc &acc, unsigned &cnt, unsigned &fetched)
+{
+    if (k0 >= k1) return;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.vox_data, (short)0, (int)a.vox_bytes, 0x00020000);
+    const unsigned esz = sizeof(VT), cy = (unsigned)a.P * esz, cz = (unsigned)(a.P * a.P) * esz;
+    auto offset = [&](const Pos &q) -> unsigned {
+        const int bid = (q.j >> a.bshift) + a.nb[0] * ((q.i >> a.bshift) + a.nb[1] * (q.k >> a.bshift));
+        return (unsigned)(s_boff[bid] + brick_elem(a.P, (q.j & a.bmask) + 1, (q.i & a.bmask) + 1,
+                                                    (q.k & a.bmask) + 1)) * esz;
+    };
+    int k = k0;
+    Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
+    float v[8];
+    fetch8_buf<VT>(rs, offset(p), cy, cz, v);
+    for (;;) {
+        const bool more = k + 1 < k1;
+        const Pos pn = cfg_pos(a, r.go, r.gd, r.tin, r.dt, more ? k + 1 : k);
+        const bool need = more && (pn.j != p.j || pn.i != p.i || pn.k != p.k);
+        float vn[8];
+        fetch8_buf<VT>(rs, need ? offset(pn) : a.vox_bytes, cy, cz, vn);
+        __builtin_amdgcn_sched_barrier(0); /* keep the requests ahead of the compositing */
+        ftb_sample(a, s_lut, v, p, acc);
+        cnt++;
+        fetched++;
+        k++;
+        if (acc.a >= a.ert || k >= k1) break;
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = need ? vn[q] : v[q];
+        p = pn;
+    }
+}
+
 /* Front-to-back march over samples [k0, k1) from acc, with early ray
  * termination.  One loop for skipping and sampling, so that within a
  * wavefront the rays crossing empty cells and the rays sampling advance
@@ -229,6 +299,10 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
                                               const unsigned long long *s_boff, const Ray &r, int k0,
                                               int k1, Acc &acc, unsigned &cnt, unsigned &fetched)
 {
+    if (!SKIP && a.vox_bytes) {
+        march_segment_pipe<VT>(a, s_lut, s_boff, r, k0, k1, acc, cnt, fetched);
+        return;
+    }
     int k = k0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     int cc = -1, e = 0;            /* macrocell of the code held in e      */
