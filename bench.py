@@ -134,8 +134,10 @@ def main():
         # one workgroup per (pixel block, run): 8 voxels per classified sample
         # read, one RGBA32F partial plane per run written
         runs = sum(vol.info()["nbricks"]) - 2
-        bytes_alg = S * 8 * vox_bytes + runs * W * H * 16
-        kernel = "k_runs<%s, %s>" % (vt, str(skip).lower())
+        # more than 4 run slots: per-ray run counts, only the existing partials written
+        counts = runs > 4
+        bytes_alg = S * 8 * vox_bytes + stats["partials"] * 16 + (4 * W * H if counts else 0)
+        kernel = "k_runs<%s, %s, %s>" % (vt, str(skip).lower(), str(counts).lower())
     else:
         bytes_alg = S * 8 * vox_bytes + 4 * W * H
         kernel = "k_render<1, %s, %s, 0>" % (vt, str(skip).lower())

@@ -104,6 +104,7 @@ typedef struct ivr_frame_stats {
     float render_ms;            /* device time of the last render's kernel(s), after
                                    the macrocell pass (events)                  */
     float composite_ms;         /* device time of its sort-last combine pass    */
+    unsigned long long partials; /* IVR_SORT_LAST: per-run partials written to HBM */
 } ivr_frame_stats;
 
 /* ---- context ---- */

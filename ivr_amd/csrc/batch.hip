@@ -79,8 +79,8 @@ __global__ __launch_bounds__(IVR_BLOCK) void k_render_batch(const RenderArgs *ar
             }
             a.out8[(size_t)i * a.W + j] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
         }
-        count_add(&a.counters[0], cnt);
-        count_add(&a.counters[1], fetched);
+        count_add(a.counters, 0, cnt);
+        count_add(a.counters, 1, fetched);
         __syncthreads();
     }
 }
@@ -163,10 +163,10 @@ int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, i
     int rc;
     for (Slot &s : b->slots) {
         if ((rc = ivr_dalloc(ctx, (void **)&s.d_img, img)) || (rc = ivr_dalloc(ctx, (void **)&s.d_lut, BATCH_LUT * 16)) ||
-            (rc = ivr_dalloc(ctx, (void **)&s.d_ctr, 16)))
+            (rc = ivr_dalloc(ctx, (void **)&s.d_ctr, IVR_CTR_BYTES)))
             return rc;
         IVR_HIP(hipHostMalloc((void **)&s.h_img, img, hipHostMallocDefault));
-        IVR_HIP(hipHostMalloc((void **)&s.h_ctr, 16, hipHostMallocDefault));
+        IVR_HIP(hipHostMalloc((void **)&s.h_ctr, IVR_CTR_BYTES, hipHostMallocDefault));
         IVR_HIP(hipHostMalloc((void **)&s.h_lut, BATCH_LUT * 16, hipHostMallocDefault));
     }
     for (int k = 0; k < 2; k++) {
@@ -192,7 +192,7 @@ int ivr_batcher_destroy(ivr_batcher *b)
     for (Slot &s : b->slots) {
         ivr_dfree(b->ctx, s.d_img, img);
         ivr_dfree(b->ctx, s.d_lut, BATCH_LUT * 16);
-        ivr_dfree(b->ctx, s.d_ctr, 16);
+        ivr_dfree(b->ctx, s.d_ctr, IVR_CTR_BYTES);
         ivr_dfree(b->ctx, s.d_cells, s.cells_cap);
         if (s.h_img) (void)hipHostFree(s.h_img);
         if (s.h_ctr) (void)hipHostFree(s.h_ctr);
@@ -296,7 +296,8 @@ int retire(ivr_batcher *b, Batch &bt, bool wait)
         c.t_submit = s.req.t_submit;
         c.t_admit = s.t_admit;
         c.t_done = t;
-        c.samples = s.h_ctr[0];
+        c.samples = 0;
+        for (int q = 0; q < IVR_CTR_SLOTS; q++) c.samples += s.h_ctr[q * IVR_CTRS];
         b->done.push_back(c);
         b->used -= s.charge;
         s.busy = false;
@@ -384,7 +385,7 @@ int launch(ivr_batcher *b)
         fr.d_rgba32 = nullptr;
         fr.d_counters = s.d_ctr;
         if (res.sg) { /* its own launch, ahead of the batch kernel on this stream */
-            IVR_HIP(hipMemsetAsync(s.d_ctr, 0, 16, st));
+            IVR_HIP(hipMemsetAsync(s.d_ctr, 0, IVR_CTR_BYTES, st));
             int rc = ivr_surrogate_launch_on(b->ctx, st, res.sg, v, &tf, &d, &fr, d.skip ? s.d_cells : nullptr,
                                              res.d_fallback);
             if (rc) return rc;
@@ -402,7 +403,7 @@ int launch(ivr_batcher *b)
             int rc = ivr_launch_cells_on(st, v, a, s.d_cells);
             if (rc) return rc;
         }
-        IVR_HIP(hipMemsetAsync(s.d_ctr, 0, 16, st));
+        IVR_HIP(hipMemsetAsync(s.d_ctr, 0, IVR_CTR_BYTES, st));
         bt->h_prefix[q2] = total;
         total += a.nblk_x * ((fr.H + 15) / 16);
         s.launched = true;
@@ -421,7 +422,7 @@ int launch(ivr_batcher *b)
         Slot &s = b->slots[si];
         const size_t bytes = (size_t)s.req.desc.state.winwidth * s.req.desc.state.winheight * 4;
         IVR_HIP(hipMemcpyAsync(s.h_img, s.d_img, bytes, hipMemcpyDeviceToHost, st));
-        IVR_HIP(hipMemcpyAsync(s.h_ctr, s.d_ctr, 16, hipMemcpyDeviceToHost, st));
+        IVR_HIP(hipMemcpyAsync(s.h_ctr, s.d_ctr, IVR_CTR_BYTES, hipMemcpyDeviceToHost, st));
     }
     IVR_HIP(hipEventRecord(bt->done, st));
     bt->active = true;

@@ -23,6 +23,13 @@
 #define IVR_FRAME_BLOCK 256 /* threads of a frame kernel's workgroup (one pixel block) */
 #endif
 #define IVR_SL_LDS (64 * IVR_FRAME_BLOCK) /* LDS for sort-last partials: 4 runs */
+/* frame counters: IVR_CTR_SLOTS stripes of IVR_CTRS u64 (samples, visited,
+ * partials, spare), a workgroup adding to stripe blockIdx % IVR_CTR_SLOTS;
+ * one address taking every wavefront's atomic was measured to slow a frame
+ * by up to 40 % */
+#define IVR_CTR_SLOTS 256
+#define IVR_CTRS 4
+#define IVR_CTR_BYTES (IVR_CTR_SLOTS * IVR_CTRS * 8)
 
 int ivr_fail(int code, const char *fmt, ...);
 #define IVR_HIP(expr)                                                                          \
@@ -82,9 +89,10 @@ struct ivr_frame {
     int W = 0, H = 0;
     uchar4 *d_rgba8 = nullptr;
     float4 *d_rgba32 = nullptr;
-    unsigned long long *d_counters = nullptr; /* [0] composited, [1] fetched */
+    unsigned long long *d_counters = nullptr; /* IVR_CTR_SLOTS stripes of IVR_CTRS */
     float4 *d_tiles = nullptr;                /* sort-last per-brick tiles    */
     size_t tiles_cap = 0;                     /* float4 elements              */
+    unsigned *d_runcount = nullptr;           /* per-run sort-last: runs per ray, W*H */
     int *d_tilemeta = nullptr;                /* sort-last tile table (device) */
     void *h_tilemeta = nullptr;               /* pinned staging of that table  */
     hipEvent_t e[3] = {};
@@ -133,6 +141,7 @@ struct RenderArgs {
     int tb[3];
     int trect[4]; /* x0, y0, w, h */
     float4 *tile;
+    unsigned *runcount; /* k_runs: runs per ray (W x H), or null: every run plane written */
 };
 
 /* Voxel order inside a brick: storage coordinates (x,y,z) in [0, P), P =

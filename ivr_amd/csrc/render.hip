@@ -68,18 +68,21 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render(const RenderArgs a)
         a.out8[px] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
         if (a.out32) a.out32[px] = make_float4(acc.r, acc.g, acc.b, acc.a);
     }
-    count_add(&a.counters[0], cnt);
-    count_add(&a.counters[1], fetched);
+    count_add(a.counters, 0, cnt);
+    count_add(a.counters, 1, fetched);
 }
 
 /* Sort-last with every run of every ray as its own work: workgroup g
  * renders run g / nblk_pad (counted front to back along each ray) of the
  * rays of pixel block g % nblk_pad, composited from zero with early
  * termination inside the run, into plane `run` of a.tile (W x H float4 per
- * plane); rays without that run write zeros.  Splitting the rays at the
- * brick faces shortens the longest chains of dependent samples, which
- * otherwise leave most of the GPU idle in the second half of a frame. */
-template <typename VT, bool SKIP>
+ * plane).  With COUNTS, a ray's run count goes to a.runcount from the
+ * workgroup of its last run (from run 0's for a ray without samples) and
+ * planes past it are neither written nor read; without, rays write zero
+ * partials for the runs they do not have.  Splitting the rays at the brick
+ * faces shortens the longest chains of dependent samples, which otherwise
+ * leave most of the GPU idle in the second half of a frame. */
+template <typename VT, bool SKIP, bool COUNTS>
 __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
 {
     extern __shared__ float4 smem[];
@@ -89,26 +92,32 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
     int i, j;
     if (!block_pixel(a, blockIdx.x - run * a.nblk_pad, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
-    unsigned cnt = 0, fetched = 0;
+    unsigned cnt = 0, fetched = 0, wrote = 0;
     if (i < a.H && j < a.W) {
         float d[3];
         cfg_dir(a, i, j, d);
         Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        march_run<VT, SKIP>(a, s_lut, s_boff, a.o, d, run, acc, cnt, fetched);
-        a.tile[((size_t)run * a.H + i) * a.W + j] = make_float4(acc.r, acc.g, acc.b, acc.a);
+        const int st = march_run<VT, SKIP>(a, s_lut, s_boff, a.o, d, run, acc, cnt, fetched);
+        wrote = (st || !COUNTS) ? 1u : 0u;
+        const size_t px = (size_t)i * a.W + j;
+        if (st || !COUNTS) a.tile[(size_t)run * a.W * a.H + px] = make_float4(acc.r, acc.g, acc.b, acc.a);
+        if (COUNTS && (st == 2 || (st == 0 && run == 0))) a.runcount[px] = (unsigned)(st ? run + 1 : 0);
     }
-    count_add(&a.counters[0], cnt);
-    count_add(&a.counters[1], fetched);
+    count_add(a.counters, 0, cnt);
+    count_add(a.counters, 1, fetched);
+    count_add(a.counters, 2, wrote);
 }
 
-/* the planes of k_runs combined far to near per pixel, then quantised */
-__global__ __launch_bounds__(256) void k_combine_runs(const float4 *part, int runs, int W, int H, uchar4 *out8,
-                                                      float4 *out32)
+/* the planes of k_runs combined far to near per pixel, then quantised;
+ * with run counts only a ray's own planes are read (its missing runs would
+ * be zero partials, which `over` leaves without effect) */
+__global__ __launch_bounds__(256) void k_combine_runs(const float4 *part, const unsigned *runcount, int runs, int W,
+                                                      int H, uchar4 *out8, float4 *out32)
 {
     const size_t np = (size_t)W * H, px = (size_t)blockIdx.x * 256 + threadIdx.x;
     if (px >= np) return;
     Acc d = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int u = runs - 1; u >= 0; u--) {
+    for (int u = (runcount ? (int)runcount[px] : runs) - 1; u >= 0; u--) {
         const float4 t = part[u * np + px];
         const Acc s = {t.x, t.y, t.z, t.w};
         over(d, s);
@@ -190,8 +199,8 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_tile(const RenderArgs a)
         }
         a.tile[(size_t)ti * a.trect[2] + tj] = make_float4(acc.r, acc.g, acc.b, acc.a);
     }
-    count_add(&a.counters[0], cnt);
-    count_add(&a.counters[1], fetched);
+    count_add(a.counters, 0, cnt);
+    count_add(a.counters, 1, fetched);
 }
 
 /* compositetiles (main.c:1203-1218): tiles sorted near -> far, composited
@@ -368,12 +377,21 @@ int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip
     const dim3 g((unsigned)a.nblk_pad * (unsigned)runs);
     const size_t sm = smem_bytes(a);
     hipStream_t s = ctx->stream;
-    if (vol->dtype == IVR_F16) {
-        if (skip) hipLaunchKernelGGL((k_runs<h16, true>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
-        else hipLaunchKernelGGL((k_runs<h16, false>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+    const dim3 b(IVR_FRAME_BLOCK);
+    if (a.runcount) {
+        if (vol->dtype == IVR_F16) {
+            if (skip) hipLaunchKernelGGL((k_runs<h16, true, true>), g, b, sm, s, a);
+            else hipLaunchKernelGGL((k_runs<h16, false, true>), g, b, sm, s, a);
+        } else {
+            if (skip) hipLaunchKernelGGL((k_runs<float, true, true>), g, b, sm, s, a);
+            else hipLaunchKernelGGL((k_runs<float, false, true>), g, b, sm, s, a);
+        }
+    } else if (vol->dtype == IVR_F16) {
+        if (skip) hipLaunchKernelGGL((k_runs<h16, true, false>), g, b, sm, s, a);
+        else hipLaunchKernelGGL((k_runs<h16, false, false>), g, b, sm, s, a);
     } else {
-        if (skip) hipLaunchKernelGGL((k_runs<float, true>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
-        else hipLaunchKernelGGL((k_runs<float, false>), g, dim3(IVR_FRAME_BLOCK), sm, s, a);
+        if (skip) hipLaunchKernelGGL((k_runs<float, true, false>), g, b, sm, s, a);
+        else hipLaunchKernelGGL((k_runs<float, false, false>), g, b, sm, s, a);
     }
     IVR_HIP(hipGetLastError());
     return IVR_OK;
@@ -382,8 +400,8 @@ int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip
 int ivr_launch_combine_runs(ivr_ctx *ctx, const RenderArgs &a, int runs)
 {
     const size_t np = (size_t)a.W * a.H;
-    hipLaunchKernelGGL(k_combine_runs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, a.tile, runs,
-                       a.W, a.H, a.out8, a.out32);
+    hipLaunchKernelGGL(k_combine_runs, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, ctx->stream, a.tile,
+                       a.runcount, runs, a.W, a.H, a.out8, a.out32);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }

@@ -507,21 +507,24 @@ __device__ __forceinline__ void march_sort_last_rev(const RenderArgs &a, const f
 }
 
 /* Run `run` (counted front to back) of one ray, composited from zero with
- * early termination inside the run; nothing when the ray has fewer runs.
- * The run's start is found by walking the boundaries before it. */
+ * early termination inside the run; its start is found by walking the
+ * boundaries before it.  Returns 0 when the ray has no such run (nothing
+ * composited), 1 for a run followed by others, 2 for the ray's last run. */
 template <typename VT, bool SKIP>
-__device__ __forceinline__ void march_run(const RenderArgs &a, const float4 *s_lut,
-                                          const unsigned long long *s_boff, const float *o, const float *d,
-                                          int run, Acc &acc, unsigned &cnt, unsigned &fetched)
+__device__ __forceinline__ int march_run(const RenderArgs &a, const float4 *s_lut,
+                                         const unsigned long long *s_boff, const float *o, const float *d,
+                                         int run, Acc &acc, unsigned &cnt, unsigned &fetched)
 {
     Ray r;
-    if (!ray_setup(a, o, d, r)) return;
+    if (!ray_setup(a, o, d, r)) return 0;
     int k = 0;
     for (int t = 0; t < run && k < r.n; t++)
         k = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
-    if (k >= r.n) return;
+    if (k >= r.n) return 0;
     const int e = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
+    const int st = e >= r.n ? 2 : 1; /* known before the march */
     march_segment<VT, SKIP>(a, s_lut, s_boff, r, k, e, acc, cnt, fetched);
+    return st;
 }
 
 /* getlinearsample, non-BGQ (ray.c:122-178): reference weights and axes */
@@ -683,11 +686,13 @@ __device__ __forceinline__ bool block_pixel(const RenderArgs &a, int b, int &i, 
     return true;
 }
 
-__device__ __forceinline__ void count_add(unsigned long long *ctr, unsigned v)
+/* adds the wavefront's sum of v to counter k of this workgroup's stripe */
+__device__ __forceinline__ void count_add(unsigned long long *ctrs, int k, unsigned v)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+    if ((threadIdx.x & 63) == 0 && v)
+        atomicAdd(&ctrs[(blockIdx.x & (IVR_CTR_SLOTS - 1)) * IVR_CTRS + k], (unsigned long long)v);
 }
 
 } // namespace ivr_dev

@@ -925,8 +925,8 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
         a.out8[px] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
         if (a.out32) a.out32[px] = make_float4(acc.r, acc.g, acc.b, acc.a);
     }
-    count_add(&a.counters[0], cnt);
-    count_add(&a.counters[1], fetched);
+    count_add(a.counters, 0, cnt);
+    count_add(a.counters, 1, fetched);
 }
 
 /* sum of squared RGBA8 differences */
@@ -939,7 +939,9 @@ __global__ void k_sqerr(const uchar4 *x, const uchar4 *y, size_t n, unsigned lon
         const int d0 = (int)u.x - w.x, d1 = (int)u.y - w.y, d2 = (int)u.z - w.z, d3 = (int)u.w - w.w;
         v = (unsigned)(d0 * d0 + d1 * d1 + d2 * d2 + d3 * d3);
     }
-    count_add(out, v);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(out, (unsigned long long)v);
 }
 
 /* the surrogate frame's kernels on stream st: macrocell codes of the frame's
@@ -1003,7 +1005,7 @@ int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr
 int sg_launch(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
               const ivr_frame_desc *d, ivr_frame *fr, const unsigned char *fallback, int only_brick)
 {
-    IVR_HIP(hipMemsetAsync(fr->d_counters, 0, 2 * sizeof(unsigned long long), ctx->stream));
+    IVR_HIP(hipMemsetAsync(fr->d_counters, 0, IVR_CTR_BYTES, ctx->stream));
     fr->timed = 1;
     fr->composited = 0;
     IVR_HIP(hipEventRecord(fr->e[0], ctx->stream));
