@@ -99,8 +99,9 @@ typedef struct ivr_frame_desc {
 } ivr_frame_desc;
 
 typedef struct ivr_frame_stats {
-    unsigned long long samples; /* samples classified and composited           */
-    unsigned long long visited; /* samples whose voxels were fetched (after skip) */
+    unsigned long long samples; /* sample positions marched, those skipped in
+                                   transparent macrocells included              */
+    unsigned long long visited; /* samples classified: voxels fetched (after skip) */
     float render_ms;            /* device time of the last render's kernel(s), after
                                    the macrocell pass (events)                  */
     float composite_ms;         /* device time of its sort-last combine pass    */
