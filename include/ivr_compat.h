@@ -76,6 +76,12 @@ float getlinearsample(float x, float y, float z);                       /* ray.c
 float getnearestsample(float x, float y, float z);                      /* ray.c:182-238 */
 void setplaneconst(float *planeconst, float *newconst);                 /* ray.c:44-51   */
 void initsamplefunc(int type);                                          /* ray.c:485-501 */
+/* voxel walk (reference ray.c:333-483, which nothing in the reference calls):
+ * entry voxel from aabb, then maxdepth offsets over volume.localdims.  On a
+ * miss (-1; only an all-zero direction) first / samples are left as they
+ * are, where the reference reads past its times[] array. */
+int getfirstvoxel(ivr_ray3f ray, ivr_ref_vector3i *first);              /* ray.c:333-384 */
+void getvoxelsonray(ivr_ray3f ray, int maxdepth, float *samples);       /* ray.c:387-483 */
 
 /* ---- transferfuncs.h:15-22 ---- */
 void inittransferfunc(int xfer);                                         /* transferfuncs.c:201-225 */

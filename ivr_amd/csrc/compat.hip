@@ -23,6 +23,7 @@
 extern "C" {
 /* host globals (main.c:63-105) */
 extern ivr_ref_volume volume;
+extern ivr_ref_aabb aabb;
 extern ivr_color4f *lut;
 extern int lutoffset;
 extern int maxlutindx;
@@ -43,7 +44,13 @@ namespace {
 
 using namespace ivr_dev;
 
-enum { OP_RAYCAST = 0, OP_SAMPLES = 1, OP_INTERSECT = 2, OP_LINEAR = 3, OP_NEAREST = 4, OP_TF = 5 };
+enum {
+    OP_RAYCAST = 0, OP_SAMPLES = 1, OP_INTERSECT = 2, OP_LINEAR = 3, OP_NEAREST = 4, OP_TF = 5,
+    OP_FIRSTVOXEL = 6, OP_VOXELS = 7
+};
+/* the voxel-walk ops read the host's aabb box and localdims from the first
+ * DDA_HEAD floats of their input, the rays after them */
+#define DDA_HEAD 12
 
 struct Compat {
     std::mutex mu;
@@ -156,6 +163,70 @@ void op_args(RenderArgs &a, int lutsize, float alpha)
     a.lutsize = lutsize;
 }
 
+/* Entry voxel of getfirstvoxel (ray.c:333-384).  box = aabb min xyz, max
+ * xyz.  The candidate parameters are box plane / direction per axis with a
+ * non-zero direction (the origin is not subtracted there), visited min x,
+ * max x, min y, ...; the first strictly smallest positive one is kept, and
+ * the entry point d*t + o is rounded half up per axis.  -1 when no candidate
+ * is positive (v untouched). */
+__device__ int first_voxel(const float *box, const float *o, const float *d, int v[3])
+{
+    float t = 0.0f;
+    int found = 0;
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const int ax = c >> 1;
+        if (d[ax] == 0.0f) continue;
+        const float tc = box[(c & 1) * 3 + ax] / d[ax];
+        if (tc > 0.0f && (t == 0.0f || tc < t)) {
+            t = tc;
+            found = 1;
+        }
+    }
+    if (!found) return -1;
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) v[ax] = (int)((d[ax] * t + o[ax]) + 0.5f);
+    return 0;
+}
+
+/* getvoxelsonray (ray.c:387-483): Amanatides-Woo walk of maxs voxels from
+ * first_voxel, writing maptoffset3d(x, y, z, w, h, d) = ((z*h + x)*w + y
+ * (mathutil.h:139-143; wrapping 32-bit) as floats.  Face parameters
+ * (boundary - o)/dir and steps sign/dir turn NaN into +inf (ray.c:436-455);
+ * the x face wins only when strictly nearest, then y over z (ray.c:462-481). */
+__device__ int voxel_walk(const float *box, const int *dims, const float *o, const float *d, int maxs,
+                          float *res)
+{
+    int v[3];
+    if (first_voxel(box, o, d, v) < 0) return -1;
+    int st[3];
+    float tm[3], dl[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+        st[ax] = d[ax] > 0.0f ? 1 : (d[ax] < 0.0f ? -1 : 0);
+        tm[ax] = ((float)(v[ax] + (st[ax] > 0)) - o[ax]) / d[ax];
+        dl[ax] = (float)st[ax] / d[ax];
+        if (tm[ax] != tm[ax]) tm[ax] = __builtin_huge_valf();
+        if (dl[ax] != dl[ax]) dl[ax] = __builtin_huge_valf();
+    }
+    const unsigned w = (unsigned)dims[0], h = (unsigned)dims[1];
+    unsigned x = (unsigned)v[0], y = (unsigned)v[1], z = (unsigned)v[2];
+    for (int i = 0; i < maxs; i++) {
+        res[i] = (float)(int)((z * h + x) * w + y);
+        if (tm[0] < tm[1] && tm[0] < tm[2]) {
+            x += (unsigned)st[0];
+            tm[0] += dl[0];
+        } else if (tm[1] < tm[2]) {
+            y += (unsigned)st[1];
+            tm[1] += dl[1];
+        } else {
+            z += (unsigned)st[2];
+            tm[2] += dl[2];
+        }
+    }
+    return 0;
+}
+
 /* per-ray / per-sample reference functions; in/out layouts per op */
 __global__ void k_ops(const RenderArgs a, int op, int n, const float *in, float *out, int maxs)
 {
@@ -206,6 +277,16 @@ __global__ void k_ops(const RenderArgs a, int op, int n, const float *in, float 
         float v[8];
         fetch8<float>(a, a.boff, j, i, k, v);
         out[t] = (float)__float_as_uint(v[0]);
+    } else if (op == OP_FIRSTVOXEL) { /* out: rc, x, y, z (integers as bits) */
+        const float *r = in + DDA_HEAD + t * 6;
+        int v[3] = {0, 0, 0};
+        const int rc = first_voxel(in, r, r + 3, v);
+        out[t * 4 + 0] = (float)rc;
+        for (int ax = 0; ax < 3; ax++) out[t * 4 + 1 + ax] = __int_as_float(v[ax]);
+    } else if (op == OP_VOXELS) { /* out: rc, then maxs offsets */
+        const float *r = in + DDA_HEAD + t * 6;
+        float *res = out + (size_t)t * (maxs + 1);
+        res[0] = (float)voxel_walk(in, (const int *)(in + 6), r, r + 3, maxs, res + 1);
     } else if (op == OP_TF) { /* one transfer function over in[0..n_s) from colour out[0..3] */
         Acc c = {out[0], out[1], out[2], out[3]};
         const int ns = maxs;
@@ -241,7 +322,10 @@ int run_op(int op, int n, const float *in, size_t in_floats, float *out, size_t 
     int rc;
     if ((rc = io_buffer(in_floats + out_floats))) return rc;
     RenderArgs a;
-    op_args(a, lutsize, alpha);
+    if (op == OP_FIRSTVOXEL || op == OP_VOXELS)
+        memset(&a, 0, sizeof a); /* the walk reads only its input */
+    else
+        op_args(a, lutsize, alpha);
     float *d_in = g.d_io, *d_out = g.d_io + in_floats;
     if (in_floats) IVR_HIP(hipMemcpyAsync(d_in, in, in_floats * sizeof(float), hipMemcpyHostToDevice, g.ctx->stream));
     if (op == OP_TF)
@@ -263,6 +347,18 @@ int prepare(int lut_entries)
 void report(int rc, const char *fn)
 {
     if (rc) fprintf(stderr, "libivr_compat: %s failed: %s\n", fn, ivr_last_error());
+}
+
+/* input head of the voxel-walk ops: aabb min, max, localdims, then the ray */
+void dda_input(const ivr_ray3f &ray, float *in)
+{
+    memset(in, 0, sizeof(float) * (DDA_HEAD + 6));
+    const float box[6] = {aabb.min.x, aabb.min.y, aabb.min.z, aabb.max.x, aabb.max.y, aabb.max.z};
+    const int dims[3] = {volume.localdims.x, volume.localdims.y, volume.localdims.z};
+    memcpy(in, box, sizeof box);
+    memcpy(in + 6, dims, sizeof dims);
+    const float r[6] = {ray.orig.x, ray.orig.y, ray.orig.z, ray.dir.x, ray.dir.y, ray.dir.z};
+    memcpy(in + DDA_HEAD, r, sizeof r);
 }
 
 /* LUT entries a transfer function touches for these samples */
@@ -361,6 +457,34 @@ bool getrayintersect(ivr_ray3f ray, float *tin, float *tout)
     *tin = o[1];
     *tout = o[2];
     return o[0] != 0.0f;
+}
+
+int getfirstvoxel(ivr_ray3f ray, ivr_ref_vector3i *first)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    float in[DDA_HEAD + 6], o[4] = {-1.0f, 0.0f, 0.0f, 0.0f};
+    dda_input(ray, in);
+    int rc = ensure_ctx();
+    if (!rc) rc = run_op(OP_FIRSTVOXEL, 1, in, DDA_HEAD + 6, o, 4, 0, 0, 0.0f);
+    report(rc, "getfirstvoxel");
+    if (rc || o[0] != 0.0f) return -1;
+    memcpy(&first->x, &o[1], sizeof(int));
+    memcpy(&first->y, &o[2], sizeof(int));
+    memcpy(&first->z, &o[3], sizeof(int));
+    return 0;
+}
+
+void getvoxelsonray(ivr_ray3f ray, int maxdepth, float *samples)
+{
+    if (maxdepth <= 0) return;
+    std::lock_guard<std::mutex> lk(g.mu);
+    float in[DDA_HEAD + 6];
+    dda_input(ray, in);
+    std::vector<float> o((size_t)maxdepth + 1, -1.0f);
+    int rc = ensure_ctx();
+    if (!rc) rc = run_op(OP_VOXELS, 1, in, DDA_HEAD + 6, o.data(), o.size(), maxdepth, 0, 0.0f);
+    report(rc, "getvoxelsonray");
+    if (!rc && o[0] == 0.0f) memcpy(samples, o.data() + 1, sizeof(float) * (size_t)maxdepth);
 }
 
 float getlinearsample(float x, float y, float z)

@@ -114,6 +114,25 @@ def box(nx, ny, nz):
     return bmin, bmax
 
 
+def firstvoxel(box6, ray):
+    """getfirstvoxel (ray.c:333-384): (rc, [x, y, z]); box6 = aabb min then max"""
+    b = np.ascontiguousarray(box6, np.float32)
+    r = np.ascontiguousarray(ray, np.float32)
+    v = (C.c_int * 3)()
+    rc = lib.ivro_firstvoxel(_f(b[:3]), _f(b[3:]), _f(r), v)
+    return rc, list(v)
+
+
+def voxelsonray(box6, dims, ray, maxdepth, fill=-7.0):
+    """getvoxelsonray (ray.c:387-483): (rc, maxdepth offsets as float32; fill where not written)"""
+    b = np.ascontiguousarray(box6, np.float32)
+    r = np.ascontiguousarray(ray, np.float32)
+    d = (C.c_int * 3)(*[int(x) for x in dims])
+    out = np.full(maxdepth, fill, np.float32)
+    rc = lib.ivro_voxelsonray(_f(b[:3]), _f(b[3:]), d, _f(r), maxdepth, _f(out))
+    return rc, out
+
+
 def ref_lib():
     """the reference compiled from its own sources, or None where absent"""
     if not os.path.exists(REF_LIB):

@@ -168,6 +168,76 @@ int ivro_rayintersect(const float bmin[3], const float bmax[3], const float nc[6
 }
 
 /* ------------------------------------------------------------------ */
+/* voxel walk (Amanatides & Woo, as in ray.c:333-483)                 */
+/* ------------------------------------------------------------------ */
+
+/* getfirstvoxel (ray.c:333-384).  Each axis with a non-zero direction
+ * offers the parameters bmin/d and bmax/d, in the order min x, max x, min y,
+ * ... (the ray origin does not enter them, ray.c:339-361); the first
+ * strictly smallest positive one is taken (ray.c:362-369) and the entry
+ * point d*t + o is rounded half up to an index per axis (ray.c:371-378).
+ * The hitface() tests there feed an array nothing reads, so they are not
+ * restated.  No positive parameter: -1 with first[] untouched, where the
+ * reference reads one element past its times[] array. */
+int ivro_firstvoxel(const float bmin[3], const float bmax[3], const float ray[6], int first[3])
+{
+    const float *o = ray, *d = ray + 3;
+    float best = 0.0f;
+    int found = 0;
+    for (int a = 0; a < 3; a++) {
+        if (d[a] == 0.0f) continue; /* both parameters are 0 there: never taken */
+        const float cand[2] = {bmin[a] / d[a], bmax[a] / d[a]};
+        for (int s = 0; s < 2; s++)
+            if (cand[s] > 0.0f && (cand[s] < best || best == 0.0f)) {
+                best = cand[s];
+                found = 1;
+            }
+    }
+    if (!found) return -1;
+    for (int a = 0; a < 3; a++) {
+        const float v = d[a] * best + o[a];
+        first[a] = (int)(v + 0.5f);
+    }
+    return 0;
+}
+
+/* getvoxelsonray (ray.c:387-483): from the entry voxel, step to the
+ * neighbour across the nearest cell face, maxdepth times, recording each
+ * voxel's maptoffset3d(x, y, z, w, h, d) = ((z*h + x)*w + y (mathutil.h:139-143,
+ * x and y in that order) converted to float.  Face parameters are
+ * (boundary - o)/d with NaN replaced by +inf (ray.c:436-455); ties go to z,
+ * then y (strict comparisons, ray.c:462-481).  The offset is formed in
+ * wrapping 32-bit arithmetic. */
+int ivro_voxelsonray(const float bmin[3], const float bmax[3], const int dims[3], const float ray[6],
+                     int maxdepth, float *samples)
+{
+    const float *o = ray, *d = ray + 3;
+    int v[3], step[3];
+    float tmax[3], delta[3];
+    if (ivro_firstvoxel(bmin, bmax, ray, v) < 0) return -1;
+    for (int a = 0; a < 3; a++) {
+        step[a] = d[a] > 0.0f ? 1 : (d[a] < 0.0f ? -1 : 0);
+        const float boundary = (float)(v[a] + (step[a] > 0 ? 1 : 0));
+        tmax[a] = (boundary - o[a]) / d[a];
+        if (isnan(tmax[a])) tmax[a] = INFINITY;
+        delta[a] = (float)step[a] / d[a];
+        if (isnan(delta[a])) delta[a] = INFINITY;
+    }
+    for (int i = 0; i < maxdepth; i++) {
+        const unsigned off = ((unsigned)v[2] * (unsigned)dims[1] + (unsigned)v[0]) * (unsigned)dims[0] +
+                             (unsigned)v[1];
+        samples[i] = (float)(int)off;
+        int a;
+        if (tmax[0] < tmax[1] && tmax[0] < tmax[2]) a = 0;
+        else if (tmax[1] < tmax[2]) a = 1;
+        else a = 2;
+        v[a] += step[a];
+        tmax[a] += delta[a];
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* samplers                                                           */
 /* ------------------------------------------------------------------ */
 

@@ -66,6 +66,14 @@ double ivro_sample_tri64(const float *vol, int nx, int ny, int nz, const double 
 int ivro_samplesonray_compat(const float *vol, int nx, int ny, int nz, const float newconst[6],
                              const float ray[6], int maxs, float *samples);
 
+/* Voxel walk of ray.c:333-483.  ivro_firstvoxel: getfirstvoxel's entry
+ * voxel (0) or -1 with first[] untouched.  ivro_voxelsonray: getvoxelsonray's
+ * maxdepth offsets maptoffset3d(x, y, z, dims) as floats, or -1 (nothing
+ * written).  bmin/bmax: the host's aabb.min/max; dims: volume.localdims. */
+int ivro_firstvoxel(const float bmin[3], const float bmax[3], const float ray[6], int first[3]);
+int ivro_voxelsonray(const float bmin[3], const float bmax[3], const int dims[3], const float ray[6],
+                     int maxdepth, float *samples);
+
 /* The six transfer functions (transferfuncs.c:24-199) over a sample array. */
 void ivro_transferfunc(int xfer, const float *lut, int lutoffset, int maxlutindx,
                        const float *samples, int n, float alpha, float out[4]);

@@ -44,6 +44,8 @@ int getsamplesonray(ray3f, int, float *);
 boolean_t getrayintersect(ray3f, float *, float *);
 float getlinearsample(float, float, float);
 void initsamplefunc(int);
+int getfirstvoxel(ray3f, vector3i *);
+void getvoxelsonray(ray3f, int, float *);
 
 /* present when linked against the drop-in libivr_compat.so */
 extern int ivr_compat_createlocalimage(int) __attribute__((weak));
@@ -162,6 +164,36 @@ int ivrref_getsamplesonray(const float r[6], int maxs, float *samples)
 {
     initsamplefunc(1);
     return getsamplesonray(mkray(r), maxs, samples);
+}
+
+/* getfirstvoxel; out[] written only on a hit (the reference's output on a
+ * miss comes from a read past its times[] array).  Callers pass rays with a
+ * non-zero direction component. */
+int ivrref_getfirstvoxel(const float r[6], int out[3])
+{
+    vector3i v;
+    int rc = getfirstvoxel(mkray(r), &v);
+    if (rc == 0) {
+        out[0] = v.x; out[1] = v.y; out[2] = v.z;
+    }
+    return rc;
+}
+
+void ivrref_getvoxelsonray(const float r[6], int maxdepth, float *samples)
+{
+    getvoxelsonray(mkray(r), maxdepth, samples);
+}
+
+/* volume.localdims, read or replaced (getvoxelsonray only forms offsets
+ * from them, so other dims exercise its index order) */
+void ivrref_localdims(int out[3])
+{
+    out[0] = volume.localdims.x; out[1] = volume.localdims.y; out[2] = volume.localdims.z;
+}
+
+void ivrref_set_localdims(const int d[3])
+{
+    volume.localdims.x = d[0]; volume.localdims.y = d[1]; volume.localdims.z = d[2];
 }
 
 void ivrref_raycast(const float r[6], float alpha, float out[4])

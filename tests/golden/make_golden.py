@@ -16,8 +16,9 @@ Vectors written (reference function -> file):
   the six transfer functions                   -> transferfuncs.npz
   the five colormaps (colormap.c)              -> colormaps.npz
   compositetiles (main.c:1203-1218)            -> composite.npz
+  getfirstvoxel + getvoxelsonray (ray.c:333-483) -> dda.npz
 
-usage: python tests/golden/make_golden.py
+usage: python tests/golden/make_golden.py [section ...]   (sections: dda; none = all)
 """
 import ctypes as C
 import hashlib
@@ -50,7 +51,55 @@ def f(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-def main():
+def dda(ref):
+    """getfirstvoxel / getvoxelsonray over rays given in world and in cell
+    coordinates (the reference's walk takes the origin as cell coordinates
+    and the entry parameters from the world box), with zero direction
+    components and grid-point origins (face ties); at the volume's own
+    localdims and at non-cubic ones, which expose the offset's index order."""
+    rng = np.random.default_rng(777)
+    rays = []
+    for k in range(120):
+        o = rng.uniform(-1.5, 1.5, 3) if k < 60 else rng.uniform(-4.0, 68.0, 3)
+        d = rng.normal(size=3) * rng.uniform(0.2, 3.0)
+        rays.append(np.concatenate([o, d]))
+    rays = np.array(rays, np.float32)
+    for k in range(6):
+        rays[k, 3 + k % 3] = 0.0
+    for k in range(6, 9):
+        rays[k, 3:] = 0.0
+        rays[k, 3 + k % 3] = (-1.0) ** k * 0.7
+    for k in range(9, 14):
+        rays[k, :3] = np.floor(rays[k, :3])
+        rays[k, 3:] = np.float32([1.0, -1.0, 1.0]) * (0.5 if k % 2 else 1.0)
+    maxdepth = 80
+    dims0 = (C.c_int * 3)()
+    ref.ivrref_localdims(dims0)
+    out = {"rays": rays, "maxdepth": maxdepth}
+    rc, first = [], []
+    for r in rays:
+        v = (C.c_int * 3)()
+        rc.append(ref.ivrref_getfirstvoxel(f(r), v))
+        first.append(list(v))
+    out["rc"] = np.array(rc, np.int32)
+    out["first"] = np.array(first, np.int32)
+    for tag, dims in (("own", tuple(dims0)), ("odd", (40, 56, 72))):
+        ref.ivrref_set_localdims((C.c_int * 3)(*dims))
+        walk = np.full((len(rays), maxdepth), -7.0, np.float32)
+        for k, r in enumerate(rays):
+            buf = np.ascontiguousarray(walk[k])
+            ref.ivrref_getvoxelsonray(f(r), maxdepth, f(buf))
+            walk[k] = buf
+        out["dims_" + tag] = np.array(dims, np.int32)
+        out["voxels_" + tag] = walk
+    ref.ivrref_set_localdims(dims0)
+    box = np.zeros(6, np.float32)
+    ref.ivrref_box(f(box))
+    out["box"] = box
+    np.savez_compressed(os.path.join(HERE, "dda.npz"), **out)
+
+
+def main(sections=()):
     ref = oracle.ref_lib()
     if ref is None:
         sys.exit("oracle/_ref/libivrref.so is not built (make -C oracle ref)")
@@ -60,6 +109,11 @@ def main():
     path = "/tmp/ivr_golden_vol.raw"
     vol.tofile(path)
     ref.ivrref_init(path.encode(), N, N, N, -1, 3, 8)
+    if sections:
+        if "dda" in sections:
+            dda(ref)
+        print("wrote", ", ".join(sections))
+        return
     box = np.zeros(6, np.float32)
     ref.ivrref_box(f(box))
     meta = dict(volume="C1 blobs seed 1001 x255", n=N, sha256=sha, width=W, height=H,
@@ -165,10 +219,12 @@ def main():
     ref.ivrref_compositetiles(f(dst), f(np.ascontiguousarray(tiles)), 5, 16, 16)
     np.savez_compressed(os.path.join(HERE, "composite.npz"), tiles=tiles, out=dst)
 
+    dda(ref)
+
     with open(os.path.join(HERE, "golden_meta.json"), "w") as fh:
         json.dump(meta, fh, indent=1)
     print("wrote golden vectors; volume sha256", sha)
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]))

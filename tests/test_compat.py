@@ -6,7 +6,7 @@ of ray.c, transferfuncs.c and colormap.c.  Driven by the same harness as the
 golden vectors, it must reproduce them bit for bit: whole frames (through the
 reference's createlocalimage calling our raycast per pixel, and through
 ivr_compat_createlocalimage as one launch), samplers, slab tests, sample
-marching, transfer functions and colormaps.
+marching, the voxel walk, transfer functions and colormaps.
 """
 import ctypes as C
 import hashlib
@@ -130,3 +130,36 @@ def test_per_ray_functions_through_dropin(refgpu):
             s = np.ascontiguousarray(z["samples"][k])
             refgpu.ivrref_transferfunc(x, f(s), 64 - k, C.c_float(float(z["alpha"])), f(o))
             assert np.array_equal(o.view(np.uint32), z["out"][x, k].view(np.uint32)), (x, k)
+
+
+@pytest.mark.gpu
+def test_voxel_walk_through_dropin(refgpu):
+    """getfirstvoxel / getvoxelsonray of the drop-in (one GPU launch per
+    call) against the compiled reference's vectors, at the volume's own
+    localdims and at non-cubic ones"""
+    z = np.load(os.path.join(G, "dda.npz"))
+    md = int(z["maxdepth"])
+    dims0 = (C.c_int * 3)()
+    refgpu.ivrref_localdims(dims0)
+    assert list(dims0) == list(z["dims_own"])
+    for k, r in enumerate(z["rays"]):
+        v = (C.c_int * 3)()
+        assert refgpu.ivrref_getfirstvoxel(f(np.ascontiguousarray(r)), v) == z["rc"][k]
+        if z["rc"][k] == 0:
+            assert list(v) == list(z["first"][k]), k
+    zero = np.float32([0.3, 0.2, 0.1, 0.0, 0.0, 0.0])  # all-zero direction: a miss, nothing written
+    v = (C.c_int * 3)(5, 6, 7)
+    assert refgpu.ivrref_getfirstvoxel(f(zero), v) == -1 and list(v) == [5, 6, 7]
+    buf = np.full(4, -7.0, np.float32)
+    refgpu.ivrref_getvoxelsonray(f(zero), 4, f(buf))
+    assert np.all(buf == -7.0)
+    try:
+        for tag in ("own", "odd"):
+            refgpu.ivrref_set_localdims((C.c_int * 3)(*[int(x) for x in z["dims_" + tag]]))
+            print("voxel walk", tag, flush=True)
+            for k, r in enumerate(z["rays"]):
+                buf = np.full(md, -7.0, np.float32)
+                refgpu.ivrref_getvoxelsonray(f(np.ascontiguousarray(r)), md, f(buf))
+                assert np.array_equal(buf.view(np.uint32), z["voxels_" + tag][k].view(np.uint32)), (tag, k)
+    finally:
+        refgpu.ivrref_set_localdims(dims0)

@@ -96,6 +96,55 @@ def test_samplesonray_bit_exact(vol255):
             assert np.array_equal(buf[:n].view(np.uint32), s[:n].view(np.uint32))
 
 
+def test_voxel_walk_bit_exact():
+    """getfirstvoxel / getvoxelsonray (ray.c:333-483) against the compiled
+    reference: 120 rays (world and cell-coordinate origins, zero direction
+    components, face ties from grid-point origins) at the volume's dims and
+    at non-cubic dims"""
+    z = np.load(os.path.join(G, "dda.npz"))
+    md = int(z["maxdepth"])
+    for k, r in enumerate(z["rays"]):
+        rc, v = oracle.firstvoxel(z["box"], r)
+        assert rc == z["rc"][k]
+        if rc == 0:
+            assert v == list(z["first"][k]), k
+        for tag in ("own", "odd"):
+            rc, w = oracle.voxelsonray(z["box"], z["dims_" + tag], r, md)
+            assert np.array_equal(w.view(np.uint32), z["voxels_" + tag][k].view(np.uint32)), (k, tag)
+
+
+def test_voxel_walk_steps_are_face_neighbours():
+    """every step of the walk moves to a face neighbour: the offset
+    ((z*h + x)*w + y changes by exactly w (x), 1 (y) or w*h (z), in the
+    direction's sign, and each axis advances at the rate 1/|d| predicts"""
+    rng = np.random.default_rng(5)
+    box = np.float32([-1, -1, -1, 1, 1, 1])
+    w, h, d = 40, 56, 72
+    for _ in range(50):
+        o = rng.uniform(0.0, 30.0, 3).astype(np.float32)
+        dr = rng.normal(size=3).astype(np.float32)
+        rc, walk = oracle.voxelsonray(box, (w, h, d), np.concatenate([o, dr]), 200)
+        assert rc == 0
+        steps = np.diff(walk.astype(np.int64))
+        allowed = {int(np.sign(dr[0])) * w, int(np.sign(dr[1])), int(np.sign(dr[2])) * w * h}
+        assert set(steps.tolist()) <= allowed
+        counts = np.array([np.sum(steps == int(np.sign(dr[0])) * w), np.sum(steps == int(np.sign(dr[1]))),
+                           np.sum(steps == int(np.sign(dr[2])) * w * h)], np.float64)
+        expect = 199 * np.abs(dr) / np.abs(dr).sum()
+        assert np.all(np.abs(counts - expect) <= 0.02 * 199 * 3 + 3), (counts, expect)
+
+
+def test_voxel_walk_zero_direction_misses():
+    box = np.float32([-1, -1, -1, 1, 1, 1])
+    rc, v = oracle.firstvoxel(box, np.float32([0.3, 0.2, 0.1, 0, 0, 0]))
+    assert rc == -1
+    rc, w = oracle.voxelsonray(box, (8, 8, 8), np.float32([0.3, 0.2, 0.1, 0, 0, 0]), 4)
+    assert rc == -1 and np.all(w == -7.0)
+    # along +x only: offsets advance by w per step
+    rc, w = oracle.voxelsonray(box, (8, 9, 10), np.float32([2.25, 3.5, 4.5, 1.0, 0, 0]), 5)
+    assert rc == 0 and np.all(np.diff(w) == 8)
+
+
 def test_transfer_functions_bit_exact():
     z = np.load(os.path.join(G, "transferfuncs.npz"))
     lut = np.ascontiguousarray(z["lut"])
