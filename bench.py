@@ -23,8 +23,9 @@ reference_cpu: the reference ray.c renderer compiled from its sources
 (oracle/_ref), single core, against this library's compat mode on the GPU
 over the same volume, camera and LUT (bit-identical images).
 
---config 4 (learned surrogate of the C2 volume) and --config 5 (steering
-service) print their own JSON lines; they are measurements of those parts,
+--config 4 (learned surrogate of the C2 volume), --config 5 (steering
+service) and --gradient (the central-difference gradient field of the C2
+volume) print their own JSON lines; they are measurements of those parts,
 not the headline.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
@@ -63,6 +64,8 @@ def main():
     ap.add_argument("--serve-surrogate", action="store_true",
                     help="--config 5: also serve the 1024^3 volume from a surrogate trained for --train-seconds")
     ap.add_argument("--requests", type=int, default=400, help="--config 5: requests per arrival rate")
+    ap.add_argument("--gradient", action="store_true",
+                    help="measure the gradient-field pass over the C2 volume instead of rendering")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (for scaling PMC byte counters)")
     args = ap.parse_args()
@@ -79,6 +82,9 @@ def main():
     import ivr_amd as ivr
     from ivr_amd import configs
 
+    if args.gradient:
+        bench_gradient(args)
+        return
     if args.config == 4:
         bench_surrogate(args)
         return
@@ -394,6 +400,64 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
     for v in vols:
         v.close()
+    ctx.close()
+
+
+def bench_gradient(args):
+    """--gradient: ivr_volume_gradient over the C2 volume (512^3 fp32, 2x2x2
+    bricks) into device memory.  HBM bound: 4 B read + 16 B written per voxel.
+    Checked bit for bit against the oracle over the whole volume; the oracle's
+    gradient on the host cores is the CPU baseline."""
+    import ctypes as C
+    import numpy as np
+    import ivr_amd as ivr
+    from ivr_amd import configs
+    import oracle
+    cfg = configs.CONFIGS[2]
+    nx, ny, nz = cfg["dims"]
+    nvox = nx * ny * nz
+    ctx = ivr.Context(0)
+    vol = ctx.volume(nx, ny, nz, cfg["dtype"], brick=cfg["brick"])
+    vol.generate(configs.gen_params(2))
+    hip = C.CDLL("libamdhip64.so")
+    dst = C.c_void_p()
+    if hip.hipMalloc(C.byref(dst), C.c_size_t(nvox * 16)) != 0:
+        raise RuntimeError("hipMalloc of the gradient field failed")
+    fn = lambda: vol.gradient(device_ptr=dst.value)  # noqa: E731
+    for _ in range(args.warmup):
+        fn()
+    ctx.sync()
+    ms = _frame_ms(ctx, fn, args.steps)
+    got = np.empty((nz, ny, nx, 4), np.float32)
+    if hip.hipMemcpy(C.c_void_p(got.ctypes.data), dst, C.c_size_t(nvox * 16), 2) != 0:  # D2H
+        raise RuntimeError("hipMemcpy of the gradient field failed")
+    hip.hipFree(dst)
+    host = vol.download()
+    t = time.perf_counter()
+    ref = oracle.gradient(host)
+    cpu_s = time.perf_counter() - t
+    same = bool(np.array_equal(got.view(np.uint32), ref.view(np.uint32)))
+    log("gradient: %.4f ms per pass, %.1f Gvoxels/s; bit-identical to the oracle: %s; oracle %.2fs"
+        % (ms, nvox / ms / 1e6, same, cpu_s))
+    vsize = 2 if cfg["dtype"] == ivr.IVR_F16 else 4
+    bytes_alg = nvox * (vsize + 16)
+    achieved = bytes_alg / (ms * 1e-3)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {
+        "metric": "Gvoxels/s, gradient field 512^3 fp32", "value": round(nvox / ms / 1e6, 3),
+        "unit": "Gvoxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "gradient field of " + cfg["name"], "voxels": nvox,
+                   "bit_identical_to_oracle": same},
+        "roofline": {"bound": "hbm", "kernel": "k_gradient<float>", "achieved": round(achieved / 1e9, 1),
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": measured_traffic("gradient field of " + cfg["name"], "gradient", False)},
+        "cpu_baseline": {"value": round(nvox / cpu_s / 1e9, 3), "unit": "Gvoxels/s", "cores": threads,
+                         "kind": "port", "sample": "the whole %dx%dx%d volume, one timed run" % (nx, ny, nz)},
+    }
+    print(json.dumps(out), flush=True)
+    vol.close()
     ctx.close()
 
 

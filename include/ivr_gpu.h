@@ -127,6 +127,14 @@ int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *vol, const float *host_xfastest)
 int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *vol, const ivr_gen_params *gp);
 int ivr_volume_download(ivr_ctx *ctx, const ivr_volume *vol, float *host_xfastest);
 int ivr_volume_destroy(ivr_ctx *ctx, ivr_volume *vol);
+/* Gradient field by central differences (replaces gradient3d, reference
+ * mathutil.c:242-328, and the GRAD block of initvolluts, main.c:1181-1196,
+ * whose volume.gradient it fills): per voxel (d/dx, d/dy, d/dz, magnitude)
+ * as 4 floats, x fastest, nx*ny*nz*16 bytes.  (f[+1] - f[-1]) / 2 inside,
+ * (f[1] - f[0]) / 2 and (f[n-1] - f[n-2]) / 2 on the faces, per axis.  dst is
+ * device memory when dst_on_device (stream-ordered, no sync), else host
+ * memory (returns when filled). */
+int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *vol, float *dst, int dst_on_device);
 /* dims[3], nbricks[3], brick edge, dtype, device bytes */
 int ivr_volume_info(const ivr_volume *vol, int dims[3], int nbricks[3], int *brick,
                     int *dtype, unsigned long long *bytes);

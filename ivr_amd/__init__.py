@@ -100,6 +100,7 @@ def _load():
         "ivr_volume_upload": (C.c_int, [P, P, P]),
         "ivr_volume_generate": (C.c_int, [P, P, C.POINTER(GenParams)]),
         "ivr_volume_download": (C.c_int, [P, P, P]),
+        "ivr_volume_gradient": (C.c_int, [P, P, P, C.c_int]),
         "ivr_volume_destroy": (C.c_int, [P, P]),
         "ivr_volume_info": (C.c_int, [P, P, P, P, P, P]),
         "ivr_tf_create": (C.c_int, [P, P, C.c_int, C.POINTER(P)]),
@@ -292,6 +293,16 @@ class Volume:
     def download(self):
         out = np.empty(self.shape, np.float32)
         check(lib.ivr_volume_download(self.ctx.h, self.h, _ptr(out)))
+        return out
+
+    def gradient(self, device_ptr=None):
+        """central-difference gradient field (nz, ny, nx, 4): d/dx, d/dy, d/dz,
+        magnitude; into device memory at device_ptr (stream-ordered) when given"""
+        if device_ptr is not None:
+            check(lib.ivr_volume_gradient(self.ctx.h, self.h, C.c_void_p(device_ptr), 1))
+            return None
+        out = np.empty(self.shape + (4,), np.float32)
+        check(lib.ivr_volume_gradient(self.ctx.h, self.h, _ptr(out), 0))
         return out
 
     def info(self):

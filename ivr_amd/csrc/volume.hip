@@ -99,6 +99,41 @@ __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *
     lin[idx] = (float)src[boff[bid] + brick_elem(g.P, l[0], l[1], l[2])];
 }
 
+/* Central-difference gradient field (the reference's volume.gradient,
+ * gradient3d mathutil.c:242-328, with its store offset i*j*k replaced by the
+ * voxel's own offset).  Per axis: (f[+1] - f[-1]) / 2 inside the volume,
+ * (f[1] - f[0]) / 2 and (f[n-1] - f[n-2]) / 2 on its two faces, each axis by
+ * its own position (the reference fills face interiors only; its last-z face
+ * takes f[n-2] - f[n-3], read here as f[n-1] - f[n-2] like the other faces).
+ * w = sqrtf(x*x + y*y + z*z) (mag3f, mathutil.c:233-236).  Output x fastest.
+ * Every neighbour read lies in the voxel's own brick: storage coordinates
+ * run over [0, B+2) and a voxel's are in [1, B+1]. */
+template <typename VT>
+__global__ void k_gradient(BrickGeom g, const unsigned long long *boff, const VT *src, float4 *out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= g.n[0]) return;
+    const int gl[3] = {x, (int)blockIdx.y, (int)blockIdx.z};
+    int b[3], l[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        b[r] = min(gl[r] >> g.bshift, g.nb[r] - 1);
+        l[r] = gl[r] - b[r] * g.B + 1;
+    }
+    const int bid = b[0] + g.nb[0] * (b[1] + g.nb[1] * b[2]);
+    const VT *p = src + boff[bid] + brick_elem(g.P, l[0], l[1], l[2]);
+    const int stride[3] = {1, g.P, g.P * g.P};
+    float d[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const int up = gl[r] < g.n[r] - 1 ? stride[r] : 0;
+        const int dn = gl[r] > 0 ? stride[r] : 0;
+        d[r] = ((float)p[up] - (float)p[-dn]) / 2.0f;
+    }
+    const float w = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    out[((size_t)gl[2] * g.n[1] + gl[1]) * g.n[0] + x] = make_float4(d[0], d[1], d[2], w);
+}
+
 /* raw generator field into fp32 bricks */
 __global__ void k_gen(BrickGeom g, const unsigned long long *boff, const GenTable *tab, float *dst)
 {
@@ -416,6 +451,30 @@ int ivr_volume_download(ivr_ctx *ctx, const ivr_volume *v, float *host)
     IVR_HIP(hipMemcpyAsync(host, lin, total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     IVR_HIP(hipStreamSynchronize(ctx->stream));
     ivr_dfree(ctx, lin, total * sizeof(float));
+    return IVR_OK;
+}
+
+int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *v, float *dst, int dst_on_device)
+{
+    if (!ctx || !v || !dst) return ivr_fail(IVR_ERR_ARG, "null argument");
+    const size_t total = (size_t)v->n[0] * v->n[1] * v->n[2];
+    float4 *out = (float4 *)dst;
+    int rc;
+    if (!dst_on_device && (rc = ivr_dalloc(ctx, (void **)&out, total * sizeof(float4)))) return rc;
+    const BrickGeom g = geom_of(v);
+    const dim3 grid((unsigned)((g.n[0] + 255) / 256), (unsigned)g.n[1], (unsigned)g.n[2]);
+    if (v->dtype == IVR_F16)
+        hipLaunchKernelGGL(k_gradient<h16>, grid, dim3(256), 0, ctx->stream, g, v->d_boff,
+                           (const h16 *)v->data, out);
+    else
+        hipLaunchKernelGGL(k_gradient<float>, grid, dim3(256), 0, ctx->stream, g, v->d_boff,
+                           (const float *)v->data, out);
+    IVR_HIP(hipGetLastError());
+    if (!dst_on_device) {
+        IVR_HIP(hipMemcpyAsync(dst, out, total * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
+        IVR_HIP(hipStreamSynchronize(ctx->stream));
+        ivr_dfree(ctx, out, total * sizeof(float4));
+    }
     return IVR_OK;
 }
 

@@ -133,6 +133,14 @@ def voxelsonray(box6, dims, ray, maxdepth, fill=-7.0):
     return rc, out
 
 
+def gradient(vol):
+    """gradient field (nz, ny, nx, 4) of a (nz, ny, nx) float32 volume"""
+    v = np.ascontiguousarray(vol, np.float32)
+    out = np.empty(v.shape + (4,), np.float32)
+    lib.ivro_gradient(_f(v), v.shape[2], v.shape[1], v.shape[0], _f(out))
+    return out
+
+
 def ref_lib():
     """the reference compiled from its own sources, or None where absent"""
     if not os.path.exists(REF_LIB):

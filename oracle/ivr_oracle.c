@@ -238,6 +238,43 @@ int ivro_voxelsonray(const float bmin[3], const float bmax[3], const int dims[3]
 }
 
 /* ------------------------------------------------------------------ */
+/* gradient field                                                     */
+/* ------------------------------------------------------------------ */
+
+/* gradient3d (mathutil.c:242-328) with grad() = the voxel at
+ * maptoffset3d(i, j, k) (mathutil.c:329-338): component x differences along
+ * j (x), y along i (y), z along k (z), each (a - b) / 2.0f * 1.0f; the
+ * interior takes the two neighbours, the faces i = 0 / m-1 (and j, k) the
+ * voxel and its inner neighbour (mathutil.c:265-326), here for every voxel
+ * and axis independently, stored at the voxel's own offset instead of the
+ * reference's i*j*k; magnitude mag3f (mathutil.c:233-236). */
+void ivro_gradient(const float *vol, int nx, int ny, int nz, float *out)
+{
+    const long long sx = 1, sy = nx, sz = (long long)nx * ny;
+#pragma omp parallel for schedule(static)
+    for (int z = 0; z < nz; z++)
+        for (int y = 0; y < ny; y++)
+            for (int x = 0; x < nx; x++) {
+                const long long o = z * sz + y * sy + x;
+                const int pos[3] = {x, y, z}, n[3] = {nx, ny, nz};
+                const long long st[3] = {sx, sy, sz};
+                float g[3];
+                for (int a = 0; a < 3; a++) {
+                    long long hi = o + st[a], lo = o - st[a];
+                    if (pos[a] == 0) lo = o;
+                    if (pos[a] == n[a] - 1) hi = o;
+                    if (n[a] == 1) hi = lo = o;
+                    g[a] = (vol[hi] - vol[lo]) / 2.0f * 1.0f;
+                }
+                float *q = out + 4 * o;
+                q[0] = g[0];
+                q[1] = g[1];
+                q[2] = g[2];
+                q[3] = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+            }
+}
+
+/* ------------------------------------------------------------------ */
 /* samplers                                                           */
 /* ------------------------------------------------------------------ */
 

@@ -74,6 +74,11 @@ int ivro_firstvoxel(const float bmin[3], const float bmax[3], const float ray[6]
 int ivro_voxelsonray(const float bmin[3], const float bmax[3], const int dims[3], const float ray[6],
                      int maxdepth, float *samples);
 
+/* Gradient field of a linear x-fastest volume (gradient3d, mathutil.c:242-328,
+ * stored at each voxel's own offset): out[4*v] = d/dx, d/dy, d/dz, magnitude;
+ * per axis (f[+1] - f[-1]) / 2 inside, one-sided differences / 2 on the faces. */
+void ivro_gradient(const float *vol, int nx, int ny, int nz, float *out);
+
 /* The six transfer functions (transferfuncs.c:24-199) over a sample array. */
 void ivro_transferfunc(int xfer, const float *lut, int lutoffset, int maxlutindx,
                        const float *samples, int n, float alpha, float out[4]);

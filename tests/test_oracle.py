@@ -145,6 +145,37 @@ def test_voxel_walk_zero_direction_misses():
     assert rc == 0 and np.all(np.diff(w) == 8)
 
 
+def test_gradient_is_numpy_gradient():
+    """the oracle's gradient field == numpy.gradient (central differences
+    (f[i+1]-f[i-1])/2 inside, one-sided first differences on the faces, here
+    halved as gradient3d divides every difference by 2), magnitude the float32
+    sqrt of (x*x + y*y) + z*z: bit for bit, on a ragged random volume"""
+    v = np.random.default_rng(1).normal(size=(7, 9, 11)).astype(np.float32)
+    g = oracle.gradient(v)
+    e = np.stack([np.gradient(v, axis=2), np.gradient(v, axis=1), np.gradient(v, axis=0)], -1)
+    for comp, axis in ((0, 2), (1, 1), (2, 0)):
+        for idx in (0, -1):
+            sl = [slice(None)] * 3
+            sl[axis] = idx
+            e[tuple(sl) + (comp,)] *= np.float32(0.5)
+    m = np.sqrt((e[..., 0] * e[..., 0] + e[..., 1] * e[..., 1]) + e[..., 2] * e[..., 2])
+    assert np.array_equal(g[..., :3].view(np.uint32), e.view(np.uint32))
+    assert np.array_equal(g[..., 3].view(np.uint32), m.view(np.uint32))
+
+
+def test_gradient_of_affine_field():
+    """f = a*x + b*y + c*z + d on the voxel grid: (a, b, c) inside, half of
+    the axis's slope on that axis's faces, the other components unchanged"""
+    z, y, x = np.meshgrid(np.arange(6), np.arange(5), np.arange(8), indexing="ij")
+    v = (0.5 * x - 0.25 * y + 2.0 * z + 1.0).astype(np.float32)
+    g = oracle.gradient(v)
+    assert np.all(g[1:-1, 1:-1, 1:-1, :3] == np.float32([0.5, -0.25, 2.0]))
+    assert np.all(g[:, :, 0, 0] == 0.25) and np.all(g[:, :, -1, 0] == 0.25)
+    assert np.all(g[:, 0, :, 1] == -0.125) and np.all(g[0, :, :, 2] == 1.0)
+    assert np.all(g[1:-1, 1:-1, 0, 1:3] == np.float32([-0.25, 2.0]))
+    assert np.allclose(g[2, 2, 2, 3], np.sqrt(0.25 + 0.0625 + 4.0))
+
+
 def test_transfer_functions_bit_exact():
     z = np.load(os.path.join(G, "transferfuncs.npz"))
     lut = np.ascontiguousarray(z["lut"])
