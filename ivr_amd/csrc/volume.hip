@@ -106,14 +106,27 @@ __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *
  * its own position (the reference fills face interiors only; its last-z face
  * takes f[n-2] - f[n-3], read here as f[n-1] - f[n-2] like the other faces).
  * w = sqrtf(x*x + y*y + z*z) (mag3f, mathutil.c:233-236).  Output x fastest.
- * Every neighbour read lies in the voxel's own brick: storage coordinates
- * run over [0, B+2) and a voxel's are in [1, B+1]. */
+ *
+ * A workgroup covers a 64 x 4 (x, y) tile, one wavefront per row, and
+ * marches GRAD_ZC slices along z keeping f(z-1), f(z), f(z+1) in registers,
+ * so a voxel row is fetched from HBM about 1.5 times (its x / y neighbours
+ * mostly come from the caches); the output is written with non-temporal
+ * stores (6 % faster than ordinary ones).  Dealing each XCD a contiguous
+ * range of tiles cut the fetched bytes from 0.80 to 0.59 GB for 512^3 but
+ * was 5 % slower, so tiles keep the hardware order.  A chunk of GRAD_ZC slices never crosses a brick face (B
+ * is a power of two >= 16); every read lies in the voxel's own brick, whose
+ * storage coordinates run over [0, B+2) while a voxel's are in [1, B+1]. */
+#define GRAD_ZC 16
 template <typename VT>
-__global__ void k_gradient(BrickGeom g, const unsigned long long *boff, const VT *src, float4 *out)
+__global__ __launch_bounds__(256) void k_gradient(BrickGeom g, const unsigned long long *boff,
+                                                  const VT *src, float4 *out)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= g.n[0]) return;
-    const int gl[3] = {x, (int)blockIdx.y, (int)blockIdx.z};
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z0 = blockIdx.z * GRAD_ZC;
+    if (x >= g.n[0] || y >= g.n[1]) return;
+    const int z1 = min(z0 + GRAD_ZC, g.n[2]);
+    const int gl[3] = {x, y, z0};
     int b[3], l[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
@@ -122,16 +135,29 @@ __global__ void k_gradient(BrickGeom g, const unsigned long long *boff, const VT
     }
     const int bid = b[0] + g.nb[0] * (b[1] + g.nb[1] * b[2]);
     const VT *p = src + boff[bid] + brick_elem(g.P, l[0], l[1], l[2]);
-    const int stride[3] = {1, g.P, g.P * g.P};
-    float d[3];
+    const int PP = g.P * g.P;
+    const int ux = x < g.n[0] - 1 ? 1 : 0, dx = x > 0 ? 1 : 0;
+    const int uy = y < g.n[1] - 1 ? g.P : 0, dy = y > 0 ? g.P : 0;
+    float fz = (float)p[0];
+    float fm = z0 > 0 ? (float)p[-PP] : fz;
+    float4 *o = out + ((size_t)z0 * g.n[1] + y) * g.n[0] + x;
+    const size_t ostep = (size_t)g.n[0] * g.n[1];
+    typedef float f4v __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        const int up = gl[r] < g.n[r] - 1 ? stride[r] : 0;
-        const int dn = gl[r] > 0 ? stride[r] : 0;
-        d[r] = ((float)p[up] - (float)p[-dn]) / 2.0f;
+    for (int k = 0; k < GRAD_ZC; k++) {
+        const int z = z0 + k;
+        if (z >= z1) break;
+        const float fp = z < g.n[2] - 1 ? (float)p[PP] : fz;
+        const float gx = ((float)p[ux] - (float)p[-dx]) / 2.0f;
+        const float gy = ((float)p[uy] - (float)p[-dy]) / 2.0f;
+        const float gz = (fp - fm) / 2.0f;
+        const f4v val = {gx, gy, gz, sqrtf(gx * gx + gy * gy + gz * gz)};
+        __builtin_nontemporal_store(val, (f4v *)o); /* streamed out, not re-read here */
+        o += ostep;
+        p += PP;
+        fm = fz;
+        fz = fp;
     }
-    const float w = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-    out[((size_t)gl[2] * g.n[1] + gl[1]) * g.n[0] + x] = make_float4(d[0], d[1], d[2], w);
 }
 
 /* raw generator field into fp32 bricks */
@@ -462,7 +488,8 @@ int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *v, float *dst, int dst_o
     int rc;
     if (!dst_on_device && (rc = ivr_dalloc(ctx, (void **)&out, total * sizeof(float4)))) return rc;
     const BrickGeom g = geom_of(v);
-    const dim3 grid((unsigned)((g.n[0] + 255) / 256), (unsigned)g.n[1], (unsigned)g.n[2]);
+    const dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4),
+                    (unsigned)((g.n[2] + GRAD_ZC - 1) / GRAD_ZC));
     if (v->dtype == IVR_F16)
         hipLaunchKernelGGL(k_gradient<h16>, grid, dim3(256), 0, ctx->stream, g, v->d_boff,
                            (const h16 *)v->data, out);
