@@ -17,8 +17,10 @@ Vectors written (reference function -> file):
   the five colormaps (colormap.c)              -> colormaps.npz
   compositetiles (main.c:1203-1218)            -> composite.npz
   getfirstvoxel + getvoxelsonray (ray.c:333-483) -> dda.npz
+  createlocalimage with cut planes (STATE newconst, setplaneconst ray.c:44-51),
+    a panned eye and a box projected past the viewport -> compat_cuts.npz
 
-usage: python tests/golden/make_golden.py [section ...]   (sections: dda; none = all)
+usage: python tests/golden/make_golden.py [section ...]   (sections: dda, cuts; none = all)
 """
 import ctypes as C
 import hashlib
@@ -99,6 +101,49 @@ def dda(ref):
     np.savez_compressed(os.path.join(HERE, "dda.npz"), **out)
 
 
+CUT_CASES = [  # (az, el, modelview z translation, x/y pan, newconst)
+    (0.7, 0.3, -5.0, (0.0, 0.0), [0.35, 0.8, 10, 0.5, 10, 0.6]),
+    (1.1, -0.2, -0.6, (0.0, 0.0), [10] * 6),
+    (1.1, -0.2, -5.0, (0.35, -0.25), [10] * 6),
+    (2.0, 0.4, -1.2, (-0.3, 0.2), [0.3, 10, 0.2, 10, 0.45, 10]),
+]
+
+
+def cut_states():
+    """STATEs of CUT_CASES: the orbit camera with its modelview translation
+    replaced and the cut planes set.  buildray takes the eye from the x/y
+    translation only (mathutil.c:1129-1137), so a pan moves the rays while
+    the z translation moves only the projected box (updatevolbbox,
+    mathutil.c:948-1018), here to a rectangle over the whole viewport."""
+    out = []
+    for az, el, tz, pan, nc in CUT_CASES:
+        st = ivr.state_from_angles(az, el, W, H)
+        st.alpha = 1.0
+        st.mvm[12], st.mvm[13], st.mvm[14] = pan[0], pan[1], tz
+        for i, v in enumerate(nc):
+            st.newconst[i] = v
+        out.append(st)
+    return out
+
+
+def cuts(ref, lut):
+    """compat frames (CHEM, the 5-point LUT) with cut planes, a panned eye
+    and a projected box larger than the viewport"""
+    imgs, states = [], []
+    ref.ivrref_set_lut(f(lut), 256, 0, 255)
+    ref.ivrref_set_xfer(3)
+    for st in cut_states():
+        out = np.zeros((H, W, 4), np.float32)
+        rect = (C.c_int * 4)()
+        ref.ivrref_render(f(np.array(st.mvm, np.float32)), f(np.array(st.pm, np.float32)),
+                          f(np.array(st.vv, np.float32)), f(np.array(st.newconst, np.float32)),
+                          C.c_float(st.alpha), W, H, f(out), rect)
+        imgs.append(out)
+        states.append(np.concatenate([st.mvm, st.pm, st.vv, st.newconst, [st.alpha]]).astype(np.float32))
+    np.savez_compressed(os.path.join(HERE, "compat_cuts.npz"), images=np.stack(imgs), states=np.stack(states),
+                        lut=lut)
+
+
 def main(sections=()):
     ref = oracle.ref_lib()
     if ref is None:
@@ -112,6 +157,8 @@ def main(sections=()):
     if sections:
         if "dda" in sections:
             dda(ref)
+        if "cuts" in sections:
+            cuts(ref, configs.tf_lut(1))
         print("wrote", ", ".join(sections))
         return
     box = np.zeros(6, np.float32)
@@ -220,6 +267,7 @@ def main(sections=()):
     np.savez_compressed(os.path.join(HERE, "composite.npz"), tiles=tiles, out=dst)
 
     dda(ref)
+    cuts(ref, luts["tf5"])
 
     with open(os.path.join(HERE, "golden_meta.json"), "w") as fh:
         json.dump(meta, fh, indent=1)

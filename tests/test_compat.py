@@ -163,3 +163,41 @@ def test_voxel_walk_through_dropin(refgpu):
                 assert np.array_equal(buf.view(np.uint32), z["voxels_" + tag][k].view(np.uint32)), (tag, k)
     finally:
         refgpu.ivrref_set_localdims(dims0)
+
+
+def _state_of(row, W, H):
+    """ivr_amd.State from a stored row: mvm[16], pm[16], vv[4], newconst[6], alpha"""
+    st = ivr.State()
+    st.winwidth, st.winheight = W, H
+    st.mvm[:] = [float(v) for v in row[0:16]]
+    st.pm[:] = [float(v) for v in row[16:32]]
+    st.vv[:] = [float(v) for v in row[32:36]]
+    st.newconst[:] = [float(v) for v in row[36:42]]
+    st.alpha = float(row[42])
+    return st
+
+
+@pytest.mark.gpu
+def test_cut_frames_through_dropin(refgpu):
+    """frames with cut planes, a panned eye and a box projected past the
+    viewport through the drop-in: the whole-frame launch for every case, the
+    per-pixel raycast path for the first"""
+    z = np.load(os.path.join(G, "compat_cuts.npz"))
+    W, H = META["width"], META["height"]
+    lut = np.ascontiguousarray(z["lut"])
+    for use_gpu_frame in (1, 0):
+        assert refgpu.ivrref_frame_on_gpu(use_gpu_frame) == use_gpu_frame
+        rows = list(enumerate(z["states"]))
+        for k, row in (rows if use_gpu_frame else rows[:1]):
+            print("drop-in cut/pan frame", use_gpu_frame, k, flush=True)
+            st = _state_of(row, W, H)
+            refgpu.ivrref_set_lut(f(lut), 256, 0, 255)
+            refgpu.ivrref_set_xfer(3)
+            out = np.zeros((H, W, 4), np.float32)
+            rect = (C.c_int * 4)()
+            rc = refgpu.ivrref_render(f(np.array(st.mvm, np.float32)), f(np.array(st.pm, np.float32)),
+                                      f(np.array(st.vv, np.float32)), f(np.array(st.newconst, np.float32)),
+                                      C.c_float(st.alpha), W, H, f(out), rect)
+            assert rc == 0
+            assert np.array_equal(out.view(np.uint32), z["images"][k].view(np.uint32)), (use_gpu_frame, k)
+    refgpu.ivrref_frame_on_gpu(0)

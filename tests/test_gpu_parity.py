@@ -95,6 +95,50 @@ def test_compat_matches_compiled_reference(ctx):
     vol.close()
 
 
+def _state_of(row, W, H):
+    """ivr_amd.State from a stored row: mvm[16], pm[16], vv[4], newconst[6], alpha"""
+    st = ivr.State()
+    st.winwidth, st.winheight = W, H
+    st.mvm[:] = [float(v) for v in row[0:16]]
+    st.pm[:] = [float(v) for v in row[16:32]]
+    st.vv[:] = [float(v) for v in row[32:36]]
+    st.newconst[:] = [float(v) for v in row[36:42]]
+    st.alpha = float(row[42])
+    return st
+
+
+def test_cut_planes_pan_and_wide_rect(ctx):
+    """STATE cut planes, a panned eye and a box projected past the viewport:
+    compat mode == the compiled reference's frames bit for bit; config mode
+    == the oracle bit for bit, fused and sort-last, skipping on and off, with
+    2^3 bricks"""
+    gp = configs.gen_params(1)
+    host = ivr.generate_host(gp, 64, 64, 64)
+    gp.value_scale = 255.0
+    host255 = ivr.generate_host(gp, 64, 64, 64)
+    z = np.load(os.path.join(G, "compat_cuts.npz"))
+    W, H = META["width"], META["height"]
+    vol255 = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    vol255.upload(host255)
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=32)
+    vol.upload(host)
+    lut = np.ascontiguousarray(z["lut"])
+    for k, row in enumerate(z["states"]):
+        print("cut/pan case", k, flush=True)
+        st = _state_of(row, W, H)
+        img8, img32, _ = _render(ctx, vol255, lut, ivr.compat_desc(st, xfer=3, maxlutindx=255), W, H)
+        assert np.array_equal(img32.view(np.uint32), z["images"][k].view(np.uint32)), k
+        ref32, s_ref = oracle.render(host, oracle.view_of(st), lut, oracle.params())
+        sl32, s_sl = _oracle_sort_last(host, st, lut, 32)
+        for skip in (False, True):
+            f8, f32, fs = _render(ctx, vol, lut, ivr.config_desc(st, skip=skip), W, H)
+            assert np.array_equal(f32.view(np.uint32), ref32.view(np.uint32)) and fs["samples"] == s_ref, (k, skip)
+            l8, l32, ls = _render(ctx, vol, lut, ivr.config_desc(st, skip=skip, composite=ivr.SORT_LAST), W, H)
+            assert np.array_equal(l32.view(np.uint32), sl32.view(np.uint32)) and ls["samples"] == s_sl, (k, skip)
+    vol.close()
+    vol255.close()
+
+
 def test_generator_gpu_equals_host(ctx):
     for (kind_cfg, n, brick) in ((1, 64, 64), (2, 96, 32)):
         gp = configs.gen_params(kind_cfg)

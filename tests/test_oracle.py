@@ -60,6 +60,33 @@ def test_compat_images_bit_exact(vol255):
         assert n > 0
 
 
+def _state_of(row, W, H):
+    """ivr_amd.State from a stored row: mvm[16], pm[16], vv[4], newconst[6], alpha"""
+    st = ivr.State()
+    st.winwidth, st.winheight = W, H
+    st.mvm[:] = [float(v) for v in row[0:16]]
+    st.pm[:] = [float(v) for v in row[16:32]]
+    st.vv[:] = [float(v) for v in row[32:36]]
+    st.newconst[:] = [float(v) for v in row[36:42]]
+    st.alpha = float(row[42])
+    return st
+
+
+def test_compat_cut_planes_pan_and_wide_rect_bit_exact(vol255):
+    """compat frames with STATE cut planes (dconst = min(box, newconst),
+    ray.c:44-51), a panned eye (buildray, mathutil.c:1129-1137) and a box
+    projected past the viewport (the pixel rectangle's one-row overflow,
+    main.c:959-962), against the compiled reference"""
+    z = np.load(os.path.join(G, "compat_cuts.npz"))
+    W, H = META["width"], META["height"]
+    for k, row in enumerate(z["states"]):
+        st = _state_of(row, W, H)
+        p = oracle.params(mode=oracle.COMPAT, xfer=3, lutscale=1.0, maxlutindx=255)
+        img, n = oracle.render(vol255, oracle.view_of(st), z["lut"], p)
+        assert np.array_equal(img.view(np.uint32), z["images"][k].view(np.uint32)), k
+        assert n > 0
+
+
 def test_sampler_bit_exact(vol255):
     z = np.load(os.path.join(G, "sampler.npz"))
     bmin, bmax = oracle.box(64, 64, 64)
