@@ -291,6 +291,22 @@ def bench_surrogate(args):
         "%d/%d bricks below 40 dB -> with fallback PSNR %.2f dB, %.3f ms"
         % (step_ms, steps, sg_ms, plain_ms, p_sg, s_sg, nfall, sg.nbricks, p_fb, fb_ms))
     flops_step = batch * 3 * 2 * (32 * 64 + 64 * 64 + 64)
+    workload = "C4 surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1" \
+        % ("C2" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks)
+    # roofline of the network kernel: per network sample 16 levels x 8 corners x 2 half-precision
+    # features gathered (512 B) and 2 x (32x64 + 64x64 + 64) = 12,416 flops, over the device time of
+    # the frame's macrocell pass + k_render_sg (HIP events)
+    k_ms, _ = kernel_ms(ctx, lambda: sg.render(vol, tf, desc, fr), fr, args.steps)
+    S = int(fr.stats()["visited"])
+    g_bytes = S * 16 * 8 * 4
+    flops = S * 2 * (32 * 64 + 64 * 64 + 64)
+    roofline = {"bound": "hbm", "kernel": "k_render_sg<true>", "kernel_ms": round(k_ms, 4),
+                "achieved": round(g_bytes / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(g_bytes / (k_ms * 1e-3) / HBM_PEAK, 4),
+                "traffic": measured_traffic(workload, "surrogate", True),
+                "mfma_tflops": round(flops / (k_ms * 1e-3) / 1e12, 2), "mfma_peak_tflops": 2500.0,
+                "mfma_frac": round(flops / (k_ms * 1e-3) / 2.5e15, 5)}
+    cpu = None if args.no_cpu else cpu_surrogate_baseline(sg, args.cpu_seconds)
     out = {
         "metric": "surrogate frames/s at 1920x1080 with per-brick fallback below 40 dB",
         "value": round(1000.0 / fb_ms, 2),
@@ -301,8 +317,7 @@ def bench_surrogate(args):
         "data": "synthetic %s volume (%s), surrogate trained on the GPU"
                 % ("C2" if c == 2 else "C3", "512^3 fp32" if c == 2 else "1024^3 fp16"),
         "config": {
-            "workload": "C4 surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1"
-                        % ("C2" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks),
+            "workload": workload,
             "image": "%dx%d" % (W, H),
             "train_steps": steps, "train_samples_per_step": batch, "train_ms_per_step": round(step_ms, 2),
             "train_mlp_tflops": round(flops_step / (step_ms * 1e-3) / 1e12, 2),
@@ -317,6 +332,8 @@ def bench_surrogate(args):
             "fallback_ms": round(fb_ms, 3), "fallback_psnr_db": round(p_fb, 2), "fallback_ssim": round(s_fb, 4),
             "plain_ms": round(plain_ms, 3), "plain_fps": round(1000.0 / plain_ms, 2),
         },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     sg.close()
@@ -324,6 +341,31 @@ def bench_surrogate(args):
     tf.close()
     vol.close()
     ctx.close()
+
+
+def cpu_surrogate_baseline(sg, target_s):
+    """The oracle's forward-only surrogate evaluation (hash encoding + MLP, fp64
+    with the bf16 rounding points) of uniform points in brick 0 on the host
+    cores, sized to about target_s seconds: the per-sample network work of a
+    surrogate frame, without the march."""
+    import numpy as np
+    import oracle
+    mlp, table = sg.get(brick=0, table=True)
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rng = np.random.default_rng(4004)
+    n = 1 << 12
+    t = time.perf_counter()
+    oracle.sg_eval(mlp, table, sg.T, rng.uniform(0.0, 1.0, (n, 3)))
+    dt = time.perf_counter() - t
+    n = int(min(1 << 24, max(n, n * target_s / max(dt, 1e-4))))
+    xyz = rng.uniform(0.0, 1.0, (n, 3))
+    t = time.perf_counter()
+    oracle.sg_eval(mlp, table, sg.T, xyz)
+    dt = time.perf_counter() - t
+    log("cpu surrogate baseline: %d points in %.2fs on %d threads" % (n, dt, threads))
+    return {"value": round(n / dt / 1e6, 3), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": "%d uniform points of brick 0 through the oracle's encoding + MLP (fp64, bf16 "
+                      "rounding), one timed run" % n}
 
 
 def bench_serving(args):

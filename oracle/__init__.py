@@ -159,6 +159,8 @@ lib.ivros_encode_backward.argtypes = [C.c_uint32, C.c_int, C.c_double, C.c_void_
 lib.ivros_mlp.restype = C.c_double
 lib.ivros_mlp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                           C.c_void_p]
+lib.ivros_eval.argtypes = [C.c_void_p, C.c_uint32, C.c_int, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                           C.c_int, C.c_void_p]
 lib.ivros_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_float, C.c_float,
                            C.c_float, C.c_float, C.c_int]
 
@@ -195,6 +197,17 @@ def sg_forward_backward(params, table, T, xyz, targets, bf16=True, nmin=16, grow
         lib.ivros_encode_backward(T, nmin, growth, _v(np.ascontiguousarray(xyz[k])),
                                   _v(np.ascontiguousarray(denc[k])), _v(gt))
     return y, loss, g, gt
+
+
+def sg_eval(params, table, T, xyz, bf16=True, nmin=16, growth=1.5):
+    """forward only (encoding + MLP) over points of one brick, OpenMP over
+    the points; float64 [n]"""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    table = np.ascontiguousarray(table, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    y = np.zeros(xyz.shape[0], np.float64)
+    lib.ivros_eval(_v(table), T, nmin, growth, _v(params), _v(xyz), xyz.shape[0], int(bf16), _v(y))
+    return y
 
 
 def sg_adam(p, g, m, v, lr, b1, b2, eps, t):

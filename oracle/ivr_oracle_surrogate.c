@@ -183,3 +183,34 @@ void ivros_adam(float *p, const float *g, float *m, float *v, long long n, float
         p[i] = (float)(p[i] - lr * (mi / c1) / (sqrt(vi / c2) + eps));
     }
 }
+
+/* Forward only: the network's value for n points of one brick (encoding
+ * then the MLP with the same bf16 rounding points as ivros_mlp's forward).
+ * The CPU baseline of the surrogate's per-sample evaluation; OpenMP over
+ * the points. */
+void ivros_eval(const float *table, uint32_t T, int nmin, double growth, const float *P, const double *xyz,
+                int n, int bf16, double *y)
+{
+    const float *W1 = P, *b1 = W1 + SG_H * SG_IN, *W2 = b1 + SG_H, *b2 = W2 + SG_H * SG_H,
+                *W3 = b2 + SG_H, *b3 = W3 + SG_H;
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < n; s++) {
+        double enc[SG_IN], e[SG_IN], h1[SG_H];
+        ivros_encode(table, T, nmin, growth, xyz + 3 * (size_t)s, enc);
+        for (int k = 0; k < SG_IN; k++) e[k] = bf16r(enc[k], bf16);
+        for (int o = 0; o < SG_H; o++) {
+            double acc = 0.0;
+            for (int k = 0; k < SG_IN; k++) acc += bf16r(W1[o * SG_IN + k], bf16) * e[k];
+            const double z = acc + b1[o];
+            h1[o] = bf16r(z > 0 ? z : 0.0, bf16);
+        }
+        double out = 0.0;
+        for (int o = 0; o < SG_H; o++) {
+            double acc = 0.0;
+            for (int k = 0; k < SG_H; k++) acc += bf16r(W2[o * SG_H + k], bf16) * h1[k];
+            const double z = acc + b2[o];
+            out += (double)W3[o] * (z > 0 ? z : 0.0);
+        }
+        y[s] = out + b3[0];
+    }
+}

@@ -65,6 +65,17 @@ def test_mlp_and_encoding_gradients_match_finite_differences():
         assert abs(fd - gt[lvl, e, f]) <= 2e-3 * max(1e-2, abs(fd)), (lvl, fd, gt[lvl, e, f])
 
 
+def test_forward_only_equals_forward_of_training_path():
+    """sg_eval (the CPU baseline's forward-only evaluation) gives exactly the
+    outputs of the forward pass inside sg_forward_backward"""
+    rng = np.random.default_rng(7)
+    table = rng.normal(0.0, 0.1, (16, T_SMALL, 2)).astype(np.float32)
+    p = _params(rng)
+    xyz = rng.uniform(0.0, 1.0, (40, 3))
+    y, _, _, _ = oracle.sg_forward_backward(p, table, T_SMALL, xyz, np.zeros(40, np.float32))
+    assert np.array_equal(oracle.sg_eval(p, table, T_SMALL, xyz), y)
+
+
 def test_adam_first_step_closed_form():
     rng = np.random.default_rng(2)
     p = rng.normal(size=100).astype(np.float32)
