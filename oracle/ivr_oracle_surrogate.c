@@ -7,9 +7,12 @@
  *   - multiresolution hash encoding (T. Mueller et al., "Instant Neural
  *     Graphics Primitives with a Multiresolution Hash Encoding", SIGGRAPH
  *     2022): level l has resolution floor(N_min * b^l); a level whose
- *     (N+1)^3 vertices fit in T entries is indexed densely, otherwise by the
- *     spatial hash (x*1 ^ y*2654435761 ^ z*805459861) mod T; features of the
- *     8 surrounding vertices are blended trilinearly;
+ *     (N+1)^3 vertices fit in T entries is indexed densely, otherwise the
+ *     4x4x4 block of vertices holding the vertex is placed by the spatial
+ *     hash (bx*1 ^ by*2654435761 ^ bz*805459861) mod T/64 of its block
+ *     coordinates into a slot of 64 entries, Morton-ordered inside (a
+ *     locality-keeping variant of the paper's per-vertex hash); features of
+ *     the 8 surrounding vertices are blended trilinearly;
  *   - MLP 32 -> 64 -> 64 -> 1 with ReLU hidden layers and a linear output,
  *     mean-squared-error loss, analytic backward pass;
  *   - Adam (D. Kingma, J. Ba, ICLR 2015) with bias correction.
@@ -51,8 +54,18 @@ uint32_t ivros_index(int l, int nmin, double growth, uint32_t T, int x, int y, i
 {
     const uint32_t n1 = (uint32_t)ivros_level_res(l, nmin, growth) + 1u;
     if ((uint64_t)n1 * n1 * n1 <= T) return (uint32_t)x + n1 * ((uint32_t)y + n1 * (uint32_t)z);
-    uint32_t h = (uint32_t)x * 1u ^ (uint32_t)y * 2654435761u ^ (uint32_t)z * 805459861u;
-    return h & (T - 1u);
+    /* hashed: block (x/4, y/4, z/4) -> slot of 64 entries by the primes
+     * 1, 2654435761, 805459861; the corner's place in the slot is the
+     * interleave x0 y0 z0 x1 y1 z1 of its low coordinate bits */
+    const uint32_t bx = (uint32_t)x / 4u, by = (uint32_t)y / 4u, bz = (uint32_t)z / 4u;
+    const uint32_t slot = (bx * 1u ^ by * 2654435761u ^ bz * 805459861u) % (T / 64u);
+    uint32_t local = 0;
+    for (int bit = 0; bit < 2; bit++) {
+        local |= (((uint32_t)x >> bit) & 1u) << (3 * bit);
+        local |= (((uint32_t)y >> bit) & 1u) << (3 * bit + 1);
+        local |= (((uint32_t)z >> bit) & 1u) << (3 * bit + 2);
+    }
+    return slot * 64u + local;
 }
 
 /* encode one point x in [0,1]^3 with table[L][T][F] (float), double result */

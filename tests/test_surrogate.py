@@ -36,6 +36,22 @@ def test_level_resolutions_and_dense_split():
     assert oracle.lib.ivros_index(4, 16, 1.5, T, 81, 81, 81) < T
 
 
+def test_block_hash_layout():
+    """hashed levels: a 4x4x4 block of vertices occupies one 64-entry slot
+    chosen by the spatial hash of its block coordinates, Morton-ordered inside
+    (x0 y0 z0 x1 y1 z1), so a cell whose corner 0 has even coordinates keeps
+    its 8 corners in 8 consecutive entries"""
+    T = 1 << 19
+    idx = lambda x, y, z: oracle.lib.ivros_index(4, 16, 1.5, T, x, y, z)  # noqa: E731
+    slot = (1 ^ 2654435761 ^ 805459861) & (T // 64 - 1)
+    assert idx(5, 6, 7) == slot * 64 + 53  # x=01 y=10 z=11 -> 1 + 4 + 16 + 32
+    block = sorted(idx(4 + a, 8 + b, 12 + c) for a in range(4) for b in range(4) for c in range(4))
+    base = ((1 ^ 2 * 2654435761 ^ 3 * 805459861) & (T // 64 - 1)) * 64
+    assert block == list(range(base, base + 64))
+    cell = sorted(idx(6 + (c & 1), 10 + (c >> 1 & 1), 14 + (c >> 2)) for c in range(8))
+    assert cell == list(range(cell[0], cell[0] + 8)) and cell[0] % 8 == 0
+
+
 def test_mlp_and_encoding_gradients_match_finite_differences():
     rng = np.random.default_rng(1)
     p = _params(rng)
@@ -54,7 +70,9 @@ def test_mlp_and_encoding_gradients_match_finite_differences():
         pm[idx] -= h
         fd = (L(pp, table) - L(pm, table)) / (2 * h)
         assert abs(fd - g[idx]) <= 2e-3 * max(1.0, abs(fd)), (idx, fd, g[idx])
-    # table entries touched by the first point at a dense and a hashed level
+    # table entries touched by the first point at a dense and a hashed level;
+    # a smaller step there keeps the perturbed encodings off the ReLU kinks
+    h = 2.0 ** -13
     for lvl in (1, 9):
         nz = np.argwhere(np.abs(gt[lvl]) > 0)
         e, f = nz[0]
