@@ -124,6 +124,9 @@ struct Batch {
     RenderArgs *d_args = nullptr, *h_args = nullptr;
     int *d_prefix = nullptr, *h_prefix = nullptr;
     unsigned *d_counter = nullptr;
+    /* surrogate requests of the batch: their frame items, rendered by one launch */
+    unsigned char *d_items = nullptr, *h_items = nullptr;
+    int *d_sgprefix = nullptr, *h_sgprefix = nullptr;
 };
 
 } // namespace
@@ -179,6 +182,12 @@ int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, i
             return rc;
         IVR_HIP(hipHostMalloc((void **)&bt.h_args, sizeof(RenderArgs) * max_slots, hipHostMallocDefault));
         IVR_HIP(hipHostMalloc((void **)&bt.h_prefix, sizeof(int) * (max_slots + 1), hipHostMallocDefault));
+        const size_t ib = ivr_surrogate_item_bytes() * max_slots;
+        if ((rc = ivr_dalloc(ctx, (void **)&bt.d_items, ib)) ||
+            (rc = ivr_dalloc(ctx, (void **)&bt.d_sgprefix, sizeof(int) * (max_slots + 1))))
+            return rc;
+        IVR_HIP(hipHostMalloc((void **)&bt.h_items, ib, hipHostMallocDefault));
+        IVR_HIP(hipHostMalloc((void **)&bt.h_sgprefix, sizeof(int) * (max_slots + 1), hipHostMallocDefault));
     }
     *out = b;
     return IVR_OK;
@@ -205,6 +214,10 @@ int ivr_batcher_destroy(ivr_batcher *b)
         ivr_dfree(b->ctx, bt.d_counter, sizeof(unsigned));
         if (bt.h_args) (void)hipHostFree(bt.h_args);
         if (bt.h_prefix) (void)hipHostFree(bt.h_prefix);
+        ivr_dfree(b->ctx, bt.d_items, ivr_surrogate_item_bytes() * b->max_slots);
+        ivr_dfree(b->ctx, bt.d_sgprefix, sizeof(int) * (b->max_slots + 1));
+        if (bt.h_items) (void)hipHostFree(bt.h_items);
+        if (bt.h_sgprefix) (void)hipHostFree(bt.h_sgprefix);
     }
     for (auto &s : b->streams) (void)hipStreamDestroy(s);
     for (Resident &r : b->res)
@@ -366,7 +379,9 @@ int launch(ivr_batcher *b)
     /* FIFO order of admission */
     std::sort(ready.begin(), ready.end(), [&](int x, int y) { return b->slots[x].t_admit < b->slots[y].t_admit; });
     hipStream_t st = b->streams[bt->stream];
-    int total = 0, nplain = 0;
+    int total = 0, nplain = 0, nsg = 0, sgtotal = 0;
+    size_t sglds = 0;
+    const size_t ib = ivr_surrogate_item_bytes();
     bt->slots = ready;
     for (size_t q = 0; q < ready.size(); q++) {
         Slot &s = b->slots[ready[q]];
@@ -384,11 +399,16 @@ int launch(ivr_batcher *b)
         fr.d_rgba8 = s.d_img;
         fr.d_rgba32 = nullptr;
         fr.d_counters = s.d_ctr;
-        if (res.sg) { /* its own launch, ahead of the batch kernel on this stream */
+        if (res.sg) { /* the batch's surrogate frames share one launch ahead of the batch kernel */
             IVR_HIP(hipMemsetAsync(s.d_ctr, 0, IVR_CTR_BYTES, st));
-            int rc = ivr_surrogate_launch_on(b->ctx, st, res.sg, v, &tf, &d, &fr, d.skip ? s.d_cells : nullptr,
-                                             res.d_fallback);
+            int nblocks = 0;
+            size_t lds = 0;
+            int rc = ivr_surrogate_prepare(b->ctx, st, res.sg, v, &tf, &d, &fr, d.skip ? s.d_cells : nullptr,
+                                           res.d_fallback, bt->h_items + ib * nsg, &nblocks, &lds);
             if (rc) return rc;
+            bt->h_sgprefix[nsg++] = sgtotal;
+            sgtotal += nblocks;
+            sglds = std::max(sglds, lds);
             s.launched = true;
             continue;
         }
@@ -407,6 +427,13 @@ int launch(ivr_batcher *b)
         bt->h_prefix[q2] = total;
         total += a.nblk_x * ((fr.H + 15) / 16);
         s.launched = true;
+    }
+    if (nsg > 0) {
+        bt->h_sgprefix[nsg] = sgtotal;
+        IVR_HIP(hipMemcpyAsync(bt->d_items, bt->h_items, ib * nsg, hipMemcpyHostToDevice, st));
+        IVR_HIP(hipMemcpyAsync(bt->d_sgprefix, bt->h_sgprefix, sizeof(int) * (nsg + 1), hipMemcpyHostToDevice, st));
+        int rc = ivr_surrogate_launch_items(st, bt->d_items, bt->d_sgprefix, nsg, sgtotal, sglds);
+        if (rc) return rc;
     }
     if (nplain > 0) {
         bt->h_prefix[nplain] = total;

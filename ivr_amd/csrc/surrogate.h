@@ -59,4 +59,17 @@ int ivr_surrogate_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s
                             const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
                             const unsigned char *fallback);
 
+/* several surrogate frames in one launch: ivr_surrogate_prepare runs the
+ * frame's macrocell pass on st and fills an item (ivr_surrogate_item_bytes()
+ * bytes, trivially copyable) with its workgroup count and LDS bytes; the
+ * caller copies the items and the prefix sums of their workgroup counts
+ * (nitems + 1 ints) to the device and calls ivr_surrogate_launch_items with
+ * the largest LDS size.  Each frame is the one ivr_surrogate_launch_on makes. */
+size_t ivr_surrogate_item_bytes(void);
+int ivr_surrogate_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                          const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                          const unsigned char *fallback, void *item, int *nblocks, size_t *lds);
+int ivr_surrogate_launch_items(hipStream_t st, const void *d_items, const int *d_prefix, int nitems, int total,
+                               size_t lds);
+
 #endif

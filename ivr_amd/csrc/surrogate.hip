@@ -823,6 +823,8 @@ struct SgRender {
     Mlp mlp;
     const unsigned char *fallback; /* per brick: 1 = fetch the plain volume */
     int only_brick;                /* >= 0: composite only that brick's samples */
+    int skip;                      /* macrocell skipping (k_render_sg_items) */
+    int nblocks;                   /* pixel-block workgroups of the frame */
 };
 
 /* Config-mode frame with the surrogate in place of the trilinear fetch.
@@ -836,14 +838,13 @@ struct SgRender {
  * of transparent macrocells (the same exact jumps as the plain renderer),
  * so only samples that can contribute reach the network. */
 template <bool SKIP>
-__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
+__device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float4 *smem)
 {
-    extern __shared__ float4 smem[];
     const RenderArgs &a = S.a;
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     int i, j;
-    if (!block_pixel(a, blockIdx.x, i, j)) return; /* uniform per workgroup */
+    if (!block_pixel(a, blk, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
     const bool inside = (i < a.H) && (j < a.W);
     float d[3] = {0.0f, 0.0f, 1.0f};
@@ -938,6 +939,28 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
     count_add(a.counters, 1, fetched);
 }
 
+template <bool SKIP>
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg(const SgRender S)
+{
+    extern __shared__ float4 smem[];
+    sg_frame_block<SKIP>(S, blockIdx.x, smem);
+}
+
+/* the frames of several requests in one grid: workgroup g renders block
+ * g - prefix[q] of item q, prefix[q] <= g < prefix[q+1]; requests of any
+ * size then share the GPU instead of running one after another */
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg_items(const SgRender *__restrict__ items,
+                                                                     const int *__restrict__ prefix, int nitems)
+{
+    extern __shared__ float4 smem[];
+    int q = 0;
+    while (q + 1 < nitems && prefix[q + 1] <= (int)blockIdx.x) q++;
+    const SgRender &S = items[q];
+    const int blk = (int)blockIdx.x - prefix[q];
+    if (S.skip) sg_frame_block<true>(S, blk, smem);
+    else sg_frame_block<false>(S, blk, smem);
+}
+
 /* sum of squared RGBA8 differences */
 __global__ void k_sqerr(const uchar4 *x, const uchar4 *y, size_t n, unsigned long long *out)
 {
@@ -956,9 +979,9 @@ __global__ void k_sqerr(const uchar4 *x, const uchar4 *y, size_t n, unsigned lon
 /* the surrogate frame's kernels on stream st: macrocell codes of the frame's
  * LUT into `cells` (with skipping), then k_render_sg; fr's counters must be
  * cleared by the caller */
-int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
-                 const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
-                 const unsigned char *fallback, int only_brick)
+int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
+               const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells, const unsigned char *fallback,
+               int only_brick, SgRender &S, size_t &lds)
 {
     if (d->mode != IVR_MODE_CONFIG) return ivr_fail(IVR_ERR_UNSUPPORTED, "surrogate renders config mode only");
     if (d->state.winwidth != fr->W || d->state.winheight != fr->H)
@@ -984,7 +1007,6 @@ int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr
     }
     ivr_cam cam;
     ivr_cam_setup(&d->state, s->n[0], s->n[1], s->n[2], &cam);
-    SgRender S;
     ivr_fill_args(S.a, cam, v, tf, d, fr);
     if (!vol) S.a.nbricks = 0; /* no brick table to stage */
     S.enc = make_enc(s);
@@ -1004,9 +1026,22 @@ int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr
         int rc = ivr_launch_cells_on(st, &geo_cells, S.a, cells);
         if (rc) return rc;
     }
-    const size_t lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
-    if (skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, st, S);
-    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.a.nblk_pad), dim3(IVR_FRAME_BLOCK), lds, st, S);
+    lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
+    S.skip = skip ? 1 : 0;
+    S.nblocks = S.a.nblk_pad;
+    return IVR_OK;
+}
+
+int sg_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                 const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                 const unsigned char *fallback, int only_brick)
+{
+    SgRender S;
+    size_t lds = 0;
+    int rc = sg_prepare(ctx, st, s, vol, tf, d, fr, cells, fallback, only_brick, S, lds);
+    if (rc) return rc;
+    if (S.skip) hipLaunchKernelGGL(k_render_sg<true>, dim3(S.nblocks), dim3(IVR_FRAME_BLOCK), lds, st, S);
+    else hipLaunchKernelGGL(k_render_sg<false>, dim3(S.nblocks), dim3(IVR_FRAME_BLOCK), lds, st, S);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }
@@ -1031,6 +1066,30 @@ int ivr_surrogate_launch_on(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s
                             const unsigned char *fallback)
 {
     return sg_launch_on(ctx, st, s, vol, tf, d, fr, cells, fallback, -1);
+}
+
+size_t ivr_surrogate_item_bytes(void) { return sizeof(SgRender); }
+
+int ivr_surrogate_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_volume *vol,
+                          const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *fr, unsigned char *cells,
+                          const unsigned char *fallback, void *item, int *nblocks, size_t *lds)
+{
+    SgRender S;
+    int rc = sg_prepare(ctx, st, s, vol, tf, d, fr, cells, fallback, -1, S, *lds);
+    if (rc) return rc;
+    memcpy(item, &S, sizeof S);
+    *nblocks = S.nblocks;
+    return IVR_OK;
+}
+
+int ivr_surrogate_launch_items(hipStream_t st, const void *d_items, const int *d_prefix, int nitems, int total,
+                               size_t lds)
+{
+    if (nitems < 1 || total < 1) return IVR_OK;
+    hipLaunchKernelGGL(k_render_sg_items, dim3(total), dim3(IVR_FRAME_BLOCK), lds, st, (const SgRender *)d_items,
+                       d_prefix, nitems);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
 }
 
 namespace {
