@@ -77,11 +77,11 @@ int ivr_ctx_create(int device, ivr_ctx **out)
     }
     if (const char *e = getenv("IVR_WAVE_W")) {
         const int w = atoi(e);
-        c->wave_w = (w == 4 || w == 8 || w == 16 || w == 32 || w == 64) ? w : 16;
+        c->wave_w = (w == 4 || w == 8 || w == 16 || w == 32 || w == 64) ? w : 0;
     }
     if (const char *e = getenv("IVR_BLOCK_W")) {
         const int w = atoi(e);
-        c->block_w = (w == 32 || w == 64) ? w : 16;
+        c->block_w = (w == 16 || w == 32 || w == 64) ? w : 0;
     }
     if (const char *e = getenv("IVR_SL_SPLIT")) c->sl_split = atoi(e) != 0;
     IVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));

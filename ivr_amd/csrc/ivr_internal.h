@@ -49,8 +49,9 @@ struct ivr_ctx {
     /* launch-shape tuning, read from IVR_XCD_CHUNK / IVR_WAVE_W / IVR_BLOCK_W at creation
      * (results are identical for every setting) */
     int xcd_chunk = IVR_CHUNK_ROW;
-    int wave_w = 16;
-    int block_w = 16; /* pixel block of a workgroup: block_w x (256 / block_w) */
+    /* 0: by the volume's storage type, 16 and 16 for fp32, 32 and 64 for fp16 */
+    int wave_w = 0;
+    int block_w = 0; /* pixel block of a workgroup: block_w x (256 / block_w) */
     int sl_split = 1; /* sort-last as one workgroup per (pixel block, run) */
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
@@ -172,8 +173,9 @@ __host__ __device__ static inline void brick_coords(int P, unsigned e, int l[3])
 void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const ivr_tf *tf,
                    const ivr_frame_desc *d, ivr_frame *f);
 
-/* grid of pixel blocks (block_w x 256/block_w) over a w x h frame with the context's chunking */
-void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h);
+/* grid of pixel blocks (block_w x 256/block_w) over a w x h frame with the context's chunking;
+ * by_type: a context left at the default shape takes the one of a.dtype_f16, else 16 x 16 */
+void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h, bool by_type = true);
 
 /* launchers (render.hip) */
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last);
