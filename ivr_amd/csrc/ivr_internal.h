@@ -174,8 +174,11 @@ void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const
                    const ivr_frame_desc *d, ivr_frame *f);
 
 /* grid of pixel blocks (block_w x 256/block_w) over a w x h frame with the context's chunking;
- * by_type: a context left at the default shape takes the one of a.dtype_f16, else 16 x 16 */
-void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h, bool by_type = true);
+ * a context left at the default shape takes the measured best one of the kernel kind: for the
+ * plain renderer by a.dtype_f16, for the surrogate 16 x 16 blocks of 8 x 8 */
+#define IVR_GRID_RENDER 0
+#define IVR_GRID_SURROGATE 1
+void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h, int kind = IVR_GRID_RENDER);
 
 /* launchers (render.hip) */
 int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip, int sort_last);

@@ -317,14 +317,16 @@ void launch_one(const RenderArgs &a, hipStream_t s, int runs = 0)
 
 } // namespace
 
-void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h, bool by_type)
+void ivr_set_grid(RenderArgs &a, const ivr_ctx *ctx, int w, int h, int kind)
 {
     /* fp16 volumes: 64 x 4 blocks of 32 x 2 wavefronts were measured faster on C3 (sort-last
      * 1.19-1.20 vs 1.24-1.26 ms with skipping, 2.00 vs 2.30 ms without; fused 1.56 vs 1.60 ms),
-     * fp32 volumes keep 16 x 16 blocks of 16 x 4 (C2 1.05 vs 1.20-1.25 ms) */
-    const bool wide = by_type && a.dtype_f16;
+     * fp32 volumes keep 16 x 16 blocks of 16 x 4 (C2 1.05 vs 1.20-1.25 ms); the surrogate's
+     * hash-grid gathers share more lines between the 8 x 8 pixels of a wavefront (C4 49.5 vs
+     * 52.2 ms with 16 x 4) */
+    const bool sg = kind == IVR_GRID_SURROGATE, wide = !sg && a.dtype_f16;
     const int bw = ctx->block_w ? ctx->block_w : (wide ? 64 : 16), bh = IVR_FRAME_BLOCK / bw;
-    const int ww = ctx->wave_w ? ctx->wave_w : (wide ? 32 : 16);
+    const int ww = ctx->wave_w ? ctx->wave_w : (wide ? 32 : sg ? 8 : 16);
     a.blk_shift = bw == 64 ? 6 : bw == 32 ? 5 : 4;
     a.nblk_x = (w + bw - 1) / bw;
     a.nblk = a.nblk_x * ((h + bh - 1) / bh);

@@ -1016,7 +1016,7 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
     S.only_brick = only_brick;
     for (int r = 0; r < 3; r++) S.a.nc[r] = s->nc[r];
     S.a.cellempty = cells;
-    ivr_set_grid(S.a, ctx, fr->W, fr->H, false);
+    ivr_set_grid(S.a, ctx, fr->W, fr->H, IVR_GRID_SURROGATE);
     const bool skip = d->skip && s->ncells > 0;
     if (skip) {
         ivr_volume geo_cells; /* the surrogate's own macrocell ranges */
