@@ -433,6 +433,34 @@ def test_c3_full_size_rows_match_oracle(ctx):
     vol.close()
 
 
+def test_volume_over_4gib_matches_oracle(ctx):
+    """a 1280^3 fp32 volume (125 bricks of 256^3, 8.6 GB of bricks: past the
+    4 GiB raw-buffer range, so the kernels take their 64-bit addressing path)
+    into a 512x512 frame with skipping; a band of rows of the fused and the
+    sort-last frame bit-exact against the oracle"""
+    n = 1280
+    vol = ctx.volume(n, n, n, ivr.IVR_F32, brick=256)
+    vol.generate(configs.gen_params(3))
+    info = vol.info()
+    assert info["nbricks"] == (5, 5, 5) and info["bytes"] > (4 << 30)
+    host = vol.download()
+    lut = configs.tf_lut(3)
+    st = configs.state(3, 512, 512)
+    rows = (248, 264)
+    sl = slice(*rows)
+    for comp in (ivr.FUSED, ivr.SORT_LAST):
+        print("over 4 GiB, composite", comp, flush=True)
+        img8, img32, _ = _render(ctx, vol, lut, ivr.config_desc(st, skip=True, composite=comp), 512, 512)
+        if comp == ivr.SORT_LAST:
+            ref32, _ = _oracle_sort_last(host, st, lut, 256, rows=rows)
+        else:
+            ref32, _ = oracle.render(host, oracle.view_of(st), lut, oracle.params(), rows=rows)
+        m, rmse, same = _report("1280^3 rows %s composite %d" % (rows, comp), img8[sl], ref32[sl], img32[sl])
+        assert same == 1.0
+        assert (img8[sl][..., 3] > 0).mean() > 0.05
+    vol.close()
+
+
 def test_error_paths(ctx, c1_host):
     vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
     vol.upload(c1_host)
