@@ -19,6 +19,8 @@
  *                          decomposition (getlocalbrick)     utils.c:332-388
  *   ivr_tf_create          the colour LUT (lut, lutoffset)   main.c:1075-1200
  *   ivr_state_orbit        the client's STATE producer       ivrclient.c:536-569
+ *   ivr_volume_gradient    gradient3d (GRAD)                 mathutil.c:242-338,
+ *                                                            main.c:1181-1196
  */
 #ifndef IVR_GPU_H
 #define IVR_GPU_H
@@ -57,7 +59,9 @@ extern "C" {
 #define IVR_SORT_LAST 1       /* the reference's sort-last scheme: one partial RGBA per
                                  brick (front to back from zero, early termination
                                  inside the brick), combined far to near with "over";
-                                 one launch, partials kept in registers */
+                                 one workgroup per pixel block and brick run writes the
+                                 partials to planes a second pass combines (IVR_SL_SPLIT=0
+                                 in the environment: one launch, partials in LDS) */
 #define IVR_SORT_LAST_TILES 2 /* the same images with the partials materialised as
                                  RGBA32F tiles in HBM (one launch per brick) and a
                                  separate "over" pass: the exchange form of a
