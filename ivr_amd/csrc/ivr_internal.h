@@ -61,6 +61,26 @@ struct ivr_ctx {
 int ivr_dalloc(ivr_ctx *ctx, void **p, size_t bytes);
 void ivr_dfree(ivr_ctx *ctx, void *p, size_t bytes);
 
+/* a temporary device buffer, freed when the scope ends (on error returns too) */
+struct ivr_dscope {
+    ivr_ctx *ctx;
+    void *p = nullptr;
+    size_t bytes = 0;
+    explicit ivr_dscope(ivr_ctx *c) : ctx(c) {}
+    ivr_dscope(const ivr_dscope &) = delete;
+    ivr_dscope &operator=(const ivr_dscope &) = delete;
+    int alloc(size_t n)
+    {
+        const int rc = ivr_dalloc(ctx, &p, n);
+        if (!rc) bytes = n;
+        return rc;
+    }
+    ~ivr_dscope()
+    {
+        if (p) ivr_dfree(ctx, p, bytes);
+    }
+};
+
 struct ivr_volume {
     int n[3] = {0, 0, 0};
     int dtype = IVR_F32;

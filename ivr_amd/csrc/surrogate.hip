@@ -738,10 +738,11 @@ int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, 
     if (!ctx || !s || n < 1 || !xyz || !bricks || !targets) return ivr_fail(IVR_ERR_ARG, "bad argument");
     for (int i = 0; i < n; i++)
         if (bricks[i] < 0 || bricks[i] >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", bricks[i]);
-    float *d = nullptr;
     const size_t bytes = (size_t)n * 4 * 6;
-    int rc = ivr_dalloc(ctx, (void **)&d, bytes);
+    ivr_dscope io(ctx);
+    int rc = io.alloc(bytes);
     if (rc) return rc;
+    float *d = (float *)io.p;
     float *dx = d, *dt = d + 3 * (size_t)n, *dy = d + 4 * (size_t)n;
     int *db = (int *)(d + 5 * (size_t)n);
     IVR_HIP(hipMemcpyAsync(dx, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
@@ -760,7 +761,6 @@ int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, 
     IVR_HIP(hipMemcpyAsync(&l, s->d_loss, 4, hipMemcpyDeviceToHost, ctx->stream));
     IVR_HIP(hipStreamSynchronize(ctx->stream));
     if (loss_out) *loss_out = l / (float)n;
-    ivr_dfree(ctx, d, bytes);
     return IVR_OK;
 }
 
@@ -1138,14 +1138,24 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
 {
     if (!ctx || !s || !vol || !tf || !d) return ivr_fail(IVR_ERR_ARG, "null argument");
     const int W = d->state.winwidth, H = d->state.winheight;
-    ivr_frame *fp = nullptr, *fs = nullptr;
-    unsigned char *dflags = nullptr;
-    unsigned long long *dsum = nullptr;
+    /* frames and buffers of the test renders, released on every return */
+    struct Frames {
+        ivr_ctx *ctx;
+        ivr_frame *f[2] = {nullptr, nullptr};
+        ~Frames()
+        {
+            for (ivr_frame *x : f)
+                if (x) ivr_frame_destroy(ctx, x);
+        }
+    } frames{ctx};
+    ivr_dscope flag_buf(ctx), sum_buf(ctx);
     int rc;
-    if ((rc = ivr_frame_create(ctx, W, H, 0, &fp)) || (rc = ivr_frame_create(ctx, W, H, 0, &fs)) ||
-        (rc = ivr_dalloc(ctx, (void **)&dflags, 2 * IVR_MAX_BRICKS)) ||
-        (rc = ivr_dalloc(ctx, (void **)&dsum, 8)))
+    if ((rc = ivr_frame_create(ctx, W, H, 0, &frames.f[0])) || (rc = ivr_frame_create(ctx, W, H, 0, &frames.f[1])) ||
+        (rc = flag_buf.alloc(2 * IVR_MAX_BRICKS)) || (rc = sum_buf.alloc(8)))
         return rc;
+    ivr_frame *fp = frames.f[0], *fs = frames.f[1];
+    unsigned char *dflags = (unsigned char *)flag_buf.p;
+    unsigned long long *dsum = (unsigned long long *)sum_buf.p;
     IVR_HIP(hipMemsetAsync(dflags, 1, IVR_MAX_BRICKS, ctx->stream));
     IVR_HIP(hipMemsetAsync(dflags + IVR_MAX_BRICKS, 0, IVR_MAX_BRICKS, ctx->stream));
     std::vector<unsigned char> flags(s->nbricks, 0);
@@ -1167,10 +1177,6 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
         flags[b] = psnr < threshold_db;
         nf += flags[b];
     }
-    ivr_frame_destroy(ctx, fp);
-    ivr_frame_destroy(ctx, fs);
-    ivr_dfree(ctx, dflags, 2 * IVR_MAX_BRICKS);
-    ivr_dfree(ctx, dsum, 8);
     if (nfallback) *nfallback = nf;
     return ivr_surrogate_set_fallback(ctx, s, flags.data());
 }

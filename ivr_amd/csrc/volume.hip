@@ -440,9 +440,10 @@ int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *v, const float *host)
 {
     if (!ctx || !v || !host) return ivr_fail(IVR_ERR_ARG, "null argument");
     const size_t total = (size_t)v->n[0] * v->n[1] * v->n[2];
-    float *lin = nullptr;
-    int rc = ivr_dalloc(ctx, (void **)&lin, total * sizeof(float));
+    ivr_dscope lin_buf(ctx);
+    int rc = lin_buf.alloc(total * sizeof(float));
     if (rc) return rc;
+    float *lin = (float *)lin_buf.p;
     IVR_HIP(hipMemcpyAsync(lin, host, total * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
     const BrickGeom g = geom_of(v);
     dim3 grid((unsigned)((g.stride + 255) / 256), (unsigned)v->nbricks);
@@ -454,7 +455,6 @@ int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *v, const float *host)
     IVR_HIP(hipGetLastError());
     if ((rc = ivr_volume_cells_build(ctx, v))) return rc;
     IVR_HIP(hipStreamSynchronize(ctx->stream));
-    ivr_dfree(ctx, lin, total * sizeof(float));
     return IVR_OK;
 }
 
@@ -462,9 +462,10 @@ int ivr_volume_download(ivr_ctx *ctx, const ivr_volume *v, float *host)
 {
     if (!ctx || !v || !host) return ivr_fail(IVR_ERR_ARG, "null argument");
     const size_t total = (size_t)v->n[0] * v->n[1] * v->n[2];
-    float *lin = nullptr;
-    int rc = ivr_dalloc(ctx, (void **)&lin, total * sizeof(float));
+    ivr_dscope lin_buf(ctx);
+    int rc = lin_buf.alloc(total * sizeof(float));
     if (rc) return rc;
+    float *lin = (float *)lin_buf.p;
     const BrickGeom g = geom_of(v);
     const unsigned nb = (unsigned)((total + 255) / 256);
     if (v->dtype == IVR_F16)
@@ -476,7 +477,6 @@ int ivr_volume_download(ivr_ctx *ctx, const ivr_volume *v, float *host)
     IVR_HIP(hipGetLastError());
     IVR_HIP(hipMemcpyAsync(host, lin, total * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
     IVR_HIP(hipStreamSynchronize(ctx->stream));
-    ivr_dfree(ctx, lin, total * sizeof(float));
     return IVR_OK;
 }
 
@@ -485,8 +485,12 @@ int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *v, float *dst, int dst_o
     if (!ctx || !v || !dst) return ivr_fail(IVR_ERR_ARG, "null argument");
     const size_t total = (size_t)v->n[0] * v->n[1] * v->n[2];
     float4 *out = (float4 *)dst;
-    int rc;
-    if (!dst_on_device && (rc = ivr_dalloc(ctx, (void **)&out, total * sizeof(float4)))) return rc;
+    ivr_dscope staging(ctx);
+    if (!dst_on_device) {
+        const int rc = staging.alloc(total * sizeof(float4));
+        if (rc) return rc;
+        out = (float4 *)staging.p;
+    }
     const BrickGeom g = geom_of(v);
     const dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4),
                     (unsigned)((g.n[2] + GRAD_ZC - 1) / GRAD_ZC));
@@ -500,7 +504,6 @@ int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *v, float *dst, int dst_o
     if (!dst_on_device) {
         IVR_HIP(hipMemcpyAsync(dst, out, total * sizeof(float4), hipMemcpyDeviceToHost, ctx->stream));
         IVR_HIP(hipStreamSynchronize(ctx->stream));
-        ivr_dfree(ctx, out, total * sizeof(float4));
     }
     return IVR_OK;
 }
@@ -511,16 +514,15 @@ int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *v, const ivr_gen_params *gp)
     GenTable tab;
     int rc = make_table(gp, &tab);
     if (rc) return rc;
-    GenTable *d_tab = nullptr;
-    unsigned *d_aux = nullptr; /* [0..1] min/max keys, [2..2049] histogram */
-    float *raw = nullptr;
     const bool inplace = v->dtype == IVR_F32;
     const size_t total_e = v->stride * v->nbricks;
-    if ((rc = ivr_dalloc(ctx, (void **)&d_tab, sizeof(GenTable))) ||
-        (rc = ivr_dalloc(ctx, (void **)&d_aux, sizeof(unsigned) * 2050)))
+    ivr_dscope tab_buf(ctx), aux_buf(ctx), raw_buf(ctx);
+    if ((rc = tab_buf.alloc(sizeof(GenTable))) || (rc = aux_buf.alloc(sizeof(unsigned) * 2050)) ||
+        (!inplace && (rc = raw_buf.alloc(total_e * sizeof(float)))))
         return rc;
-    if (inplace) raw = (float *)v->data;
-    else if ((rc = ivr_dalloc(ctx, (void **)&raw, total_e * sizeof(float)))) return rc;
+    GenTable *d_tab = (GenTable *)tab_buf.p;
+    unsigned *d_aux = (unsigned *)aux_buf.p; /* [0..1] min/max keys, [2..2049] histogram */
+    float *raw = inplace ? (float *)v->data : (float *)raw_buf.p;
     IVR_HIP(hipMemcpyAsync(d_tab, &tab, sizeof(GenTable), hipMemcpyHostToDevice, ctx->stream));
     const BrickGeom g = geom_of(v);
     dim3 grid((unsigned)((g.stride + 255) / 256), (unsigned)v->nbricks);
@@ -572,9 +574,6 @@ int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *v, const ivr_gen_params *gp)
     IVR_HIP(hipGetLastError());
     if ((rc = ivr_volume_cells_build(ctx, v))) return rc;
     IVR_HIP(hipStreamSynchronize(ctx->stream));
-    if (!inplace) ivr_dfree(ctx, raw, total_e * sizeof(float));
-    ivr_dfree(ctx, d_tab, sizeof(GenTable));
-    ivr_dfree(ctx, d_aux, sizeof(unsigned) * 2050);
     return IVR_OK;
 }
 
