@@ -148,6 +148,10 @@ struct ivr_batcher {
 
 double ivr_now(void) { return now_s(); }
 
+namespace {
+int batcher_init(ivr_batcher *b);
+}
+
 int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, int max_w, int max_h,
                        ivr_batcher **out)
 {
@@ -160,9 +164,25 @@ int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, i
     b->max_w = max_w;
     b->max_h = max_h;
     b->st.budget = budget;
+    const int rc = batcher_init(b);
+    if (rc) {
+        ivr_batcher_destroy(b); /* releases what was allocated */
+        return rc;
+    }
+    *out = b;
+    return IVR_OK;
+}
+
+namespace {
+
+/* streams, slot and batch buffers of a new batcher */
+int batcher_init(ivr_batcher *b)
+{
+    ivr_ctx *ctx = b->ctx;
+    const int max_slots = b->max_slots;
     for (auto &s : b->streams) IVR_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     b->slots.resize(max_slots);
-    const size_t img = (size_t)max_w * max_h * 4;
+    const size_t img = (size_t)b->max_w * b->max_h * 4;
     int rc;
     for (Slot &s : b->slots) {
         if ((rc = ivr_dalloc(ctx, (void **)&s.d_img, img)) || (rc = ivr_dalloc(ctx, (void **)&s.d_lut, BATCH_LUT * 16)) ||
@@ -189,14 +209,16 @@ int ivr_batcher_create(ivr_ctx *ctx, unsigned long long budget, int max_slots, i
         IVR_HIP(hipHostMalloc((void **)&bt.h_items, ib, hipHostMallocDefault));
         IVR_HIP(hipHostMalloc((void **)&bt.h_sgprefix, sizeof(int) * (max_slots + 1), hipHostMallocDefault));
     }
-    *out = b;
     return IVR_OK;
 }
+
+} // namespace
 
 int ivr_batcher_destroy(ivr_batcher *b)
 {
     if (!b) return IVR_OK;
-    for (auto &s : b->streams) (void)hipStreamSynchronize(s);
+    for (auto &s : b->streams)
+        if (s) (void)hipStreamSynchronize(s);
     const size_t img = (size_t)b->max_w * b->max_h * 4;
     for (Slot &s : b->slots) {
         ivr_dfree(b->ctx, s.d_img, img);
@@ -219,7 +241,8 @@ int ivr_batcher_destroy(ivr_batcher *b)
         if (bt.h_items) (void)hipHostFree(bt.h_items);
         if (bt.h_sgprefix) (void)hipHostFree(bt.h_sgprefix);
     }
-    for (auto &s : b->streams) (void)hipStreamDestroy(s);
+    for (auto &s : b->streams)
+        if (s) (void)hipStreamDestroy(s);
     for (Resident &r : b->res)
         if (r.sg) ivr_dfree(b->ctx, r.d_fallback, r.sg->nbricks);
     delete b;
