@@ -476,6 +476,19 @@ def test_error_paths(ctx, c1_host):
     fr.close()
     tf.close()
     vol.close()
+    # allocations that cannot succeed (TiB-sized) fail with IVR_ERR_NOMEM and release what the
+    # call had already allocated
+    before = ctx.device_bytes()
+    with pytest.raises(ivr.IvrError, match="hipMalloc"):
+        ctx.batcher(1 << 40, 2, 1 << 20, 1 << 20)
+    assert ctx.device_bytes() == before
+    with pytest.raises(ivr.IvrError, match="hipMalloc"):
+        ctx.volume(8192, 8192, 8192, ivr.IVR_F32, brick=512)
+    assert ctx.device_bytes() == before
+    small = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)  # the context still works
+    small.upload(c1_host)
+    small.close()
+    assert ctx.device_bytes() == before
 
 
 @pytest.mark.parametrize("dims,f16,brick", [((37, 50, 29), False, 16), ((17, 17, 17), False, 16),
