@@ -44,6 +44,8 @@ def _load():
     lib.ivro_sample_ref.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                     C.c_float, C.c_float, C.c_float]
     lib.ivro_sample_tri.argtypes = lib.ivro_sample_ref.argtypes
+    lib.ivro_sample_nearest.restype = C.c_float
+    lib.ivro_sample_nearest.argtypes = lib.ivro_sample_ref.argtypes
     return lib
 
 

@@ -280,6 +280,25 @@ void ivro_gradient(const float *vol, int nx, int ny, int nz, float *out)
 
 #define VOX(v, x, y, z, nx, ny) ((v)[((size_t)(z) * (size_t)(ny) + (size_t)(y)) * (size_t)(nx) + (size_t)(x)])
 
+/* getnearestsample, non-BGQ branch (ray.c:225-237): index = (int)(n * ((p - min) / extent))
+ * per axis, clamped to n-2 above (and 0 below, where the reference reads before the array);
+ * the voxel's bit pattern as an unsigned int (volume.data.i, types.h:139-141) converted to
+ * float */
+float ivro_sample_nearest(const float *vol, int nx, int ny, int nz, const float bmin[3],
+                          const float bmax[3], float x, float y, float z)
+{
+    const float w = bmax[0] - bmin[0], h = bmax[1] - bmin[1], d = bmax[2] - bmin[2];
+    int j = (int)((float)nx * ((x - bmin[0]) / w));
+    int i = (int)((float)ny * ((y - bmin[1]) / h));
+    int k = (int)((float)nz * ((z - bmin[2]) / d));
+    i = i >= ny - 2 ? ny - 2 : (i < 0 ? 0 : i);
+    j = j >= nx - 2 ? nx - 2 : (j < 0 ? 0 : j);
+    k = k >= nz - 2 ? nz - 2 : (k < 0 ? 0 : k);
+    uint32_t bits;
+    memcpy(&bits, &VOX(vol, j, i, k, nx, ny), sizeof bits);
+    return (float)bits;
+}
+
 /* getlinearsample, non-BGQ branch (ray.c:122-178), including its weight
  * scale (ext/N instead of N/ext) and the x-weight blending the +y
  * neighbour.  Deviation: indices are clamped below at 0, where the

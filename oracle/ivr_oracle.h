@@ -59,6 +59,11 @@ float ivro_sample_ref(const float *vol, int nx, int ny, int nz, const float bmin
                       const float bmax[3], float x, float y, float z);
 float ivro_sample_tri(const float *vol, int nx, int ny, int nz, const float bmin[3],
                       const float bmax[3], float x, float y, float z);
+/* getnearestsample (ray.c:182-238, non-BGQ): the voxel at the truncated
+ * index (clamped to n-2 above, 0 below), its bits read as an unsigned int and
+ * converted to float. */
+float ivro_sample_nearest(const float *vol, int nx, int ny, int nz, const float bmin[3],
+                          const float bmax[3], float x, float y, float z);
 double ivro_sample_tri64(const float *vol, int nx, int ny, int nz, const double bmin[3],
                          const double bmax[3], double x, double y, double z);
 

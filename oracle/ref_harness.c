@@ -43,6 +43,7 @@ void raycast(ray3f, color4f *);
 int getsamplesonray(ray3f, int, float *);
 boolean_t getrayintersect(ray3f, float *, float *);
 float getlinearsample(float, float, float);
+float getnearestsample(float, float, float);
 void initsamplefunc(int);
 int getfirstvoxel(ray3f, vector3i *);
 void getvoxelsonray(ray3f, int, float *);
@@ -154,6 +155,7 @@ static ray3f mkray(const float r[6])
 }
 
 float ivrref_getlinearsample(float x, float y, float z) { return getlinearsample(x, y, z); }
+float ivrref_getnearestsample(float x, float y, float z) { return getnearestsample(x, y, z); }
 
 int ivrref_getrayintersect(const float r[6], float *tin, float *tout)
 {

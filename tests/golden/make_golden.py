@@ -17,10 +17,11 @@ Vectors written (reference function -> file):
   the five colormaps (colormap.c)              -> colormaps.npz
   compositetiles (main.c:1203-1218)            -> composite.npz
   getfirstvoxel + getvoxelsonray (ray.c:333-483) -> dda.npz
+  getnearestsample (ray.c:182-238)             -> nearest.npz
   createlocalimage with cut planes (STATE newconst, setplaneconst ray.c:44-51),
     a panned eye and a box projected past the viewport -> compat_cuts.npz
 
-usage: python tests/golden/make_golden.py [section ...]   (sections: dda, cuts; none = all)
+usage: python tests/golden/make_golden.py [section ...]   (sections: dda, cuts, nearest; none = all)
 """
 import ctypes as C
 import hashlib
@@ -144,6 +145,20 @@ def cuts(ref, lut):
                         lut=lut)
 
 
+def nearest(ref):
+    """getnearestsample at points inside the box and on its faces (the
+    reference has no lower index clamp, so no point lies below the box)"""
+    rng = np.random.default_rng(778)
+    pts = rng.uniform(-1.0, 1.0, (200, 3)).astype(np.float32)
+    pts[:20, 0] = -1.0
+    pts[20:40, 1] = 1.0
+    pts[40:60, 2] = 0.999
+    ref.ivrref_getnearestsample.restype = C.c_float
+    vals = np.array([ref.ivrref_getnearestsample(C.c_float(p[0]), C.c_float(p[1]), C.c_float(p[2]))
+                     for p in pts], np.float32)
+    np.savez_compressed(os.path.join(HERE, "nearest.npz"), points=pts, values=vals)
+
+
 def main(sections=()):
     ref = oracle.ref_lib()
     if ref is None:
@@ -159,6 +174,8 @@ def main(sections=()):
             dda(ref)
         if "cuts" in sections:
             cuts(ref, configs.tf_lut(1))
+        if "nearest" in sections:
+            nearest(ref)
         print("wrote", ", ".join(sections))
         return
     box = np.zeros(6, np.float32)
@@ -268,6 +285,7 @@ def main(sections=()):
 
     dda(ref)
     cuts(ref, luts["tf5"])
+    nearest(ref)
 
     with open(os.path.join(HERE, "golden_meta.json"), "w") as fh:
         json.dump(meta, fh, indent=1)

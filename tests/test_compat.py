@@ -50,6 +50,7 @@ def _load_refgpu():
         pytest.skip("oracle/_ref/libivrref_gpu.so not built (needs the reference sources)")
     lib = C.CDLL(REFGPU)
     lib.ivrref_getlinearsample.restype = C.c_float
+    lib.ivrref_getnearestsample.restype = C.c_float
     return lib
 
 
@@ -107,6 +108,10 @@ def test_frames_through_dropin(refgpu):
 def test_per_ray_functions_through_dropin(refgpu):
     z = np.load(os.path.join(G, "sampler.npz"))
     vals = np.array([refgpu.ivrref_getlinearsample(C.c_float(p[0]), C.c_float(p[1]), C.c_float(p[2]))
+                     for p in z["points"][:60]], np.float32)
+    assert np.array_equal(vals.view(np.uint32), z["values"][:60].view(np.uint32))
+    z = np.load(os.path.join(G, "nearest.npz"))
+    vals = np.array([refgpu.ivrref_getnearestsample(C.c_float(p[0]), C.c_float(p[1]), C.c_float(p[2]))
                      for p in z["points"][:60]], np.float32)
     assert np.array_equal(vals.view(np.uint32), z["values"][:60].view(np.uint32))
     z = np.load(os.path.join(G, "slab.npz"))

@@ -97,6 +97,18 @@ def test_sampler_bit_exact(vol255):
     assert np.array_equal(got.view(np.uint32), z["values"].view(np.uint32))
 
 
+def test_nearest_sampler_bit_exact(vol255):
+    """getnearestsample (ray.c:182-238): the voxel's bits as an unsigned int,
+    as float, at 200 points against the compiled reference"""
+    z = np.load(os.path.join(G, "nearest.npz"))
+    bmin, bmax = oracle.box(64, 64, 64)
+    vals = np.array([oracle.lib.ivro_sample_nearest(vol255.ctypes.data_as(C.c_void_p), 64, 64, 64,
+                                                     bmin.ctypes.data_as(C.c_void_p),
+                                                     bmax.ctypes.data_as(C.c_void_p), *map(float, p))
+                     for p in z["points"]], np.float32)
+    assert np.array_equal(vals.view(np.uint32), z["values"].view(np.uint32))
+
+
 def test_slab_bit_exact():
     z = np.load(os.path.join(G, "slab.npz"))
     bmin, bmax = oracle.box(64, 64, 64)
