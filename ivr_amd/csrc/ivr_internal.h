@@ -114,6 +114,8 @@ struct ivr_frame {
     float4 *d_tiles = nullptr;                /* sort-last per-brick tiles    */
     size_t tiles_cap = 0;                     /* float4 elements              */
     unsigned *d_runcount = nullptr;           /* per-run sort-last: runs per ray, W*H */
+    int *d_blockmax = nullptr;                /* and the most runs of a ray per pixel block */
+    size_t blockmax_cap = 0;                  /* ints */
     int *d_tilemeta = nullptr;                /* sort-last tile table (device) */
     void *h_tilemeta = nullptr;               /* pinned staging of that table  */
     hipEvent_t e[3] = {};
@@ -163,6 +165,7 @@ struct RenderArgs {
     int trect[4]; /* x0, y0, w, h */
     float4 *tile;
     unsigned *runcount; /* k_runs: runs per ray (W x H), or null: every run plane written */
+    int *blockmax;      /* with run counts: the most runs of a ray of each pixel block */
 };
 
 /* Voxel order inside a brick: storage coordinates (x,y,z) in [0, P), P =
@@ -208,6 +211,8 @@ int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderA
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a);
 /* sort-last with one workgroup per (pixel block, run): partial planes into a.tile, then combined */
 int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);
+/* a.blockmax of every pixel block (a.nblk_pad ints) ahead of k_runs with run counts */
+int ivr_launch_blockruns(ivr_ctx *ctx, RenderArgs &a);
 int ivr_launch_combine_runs(ivr_ctx *ctx, const RenderArgs &a, int runs);
 int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles);
 

@@ -89,6 +89,8 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
     float4 *s_lut = smem;
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     const int run = blockIdx.x / a.nblk_pad;
+    /* with run counts, a run no ray of the block has is skipped whole (its rays write nothing) */
+    if (COUNTS && run > 0 && run >= a.blockmax[blockIdx.x - run * a.nblk_pad]) return;
     int i, j;
     if (!block_pixel(a, blockIdx.x - run * a.nblk_pad, i, j)) return; /* uniform per workgroup */
     stage(a, s_lut, s_boff);
@@ -106,6 +108,33 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
     count_add(a.counters, 0, cnt);
     count_add(a.counters, 1, fetched);
     count_add(a.counters, 2, wrote);
+}
+
+/* the most runs of a ray of each pixel block, counted by the boundary walk
+ * k_runs uses (run_end from the first sample until the ray's end) */
+__global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_blockruns(const RenderArgs a)
+{
+    int i, j;
+    if (!block_pixel(a, blockIdx.x, i, j)) return; /* uniform per workgroup */
+    int cnt = 0;
+    if (i < a.H && j < a.W) {
+        float d[3];
+        cfg_dir(a, i, j, d);
+        Ray r;
+        if (ray_setup(a, a.o, d, r))
+            for (int k = 0; k < r.n; cnt++)
+                k = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
+    __shared__ int s_m[IVR_FRAME_BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = 0;
+        for (int w = 0; w < IVR_FRAME_BLOCK / 64; w++) m = max(m, s_m[w]);
+        a.blockmax[blockIdx.x] = m;
+    }
 }
 
 /* the planes of k_runs combined far to near per pixel, then quantised;
@@ -400,6 +429,14 @@ int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip
         if (skip) hipLaunchKernelGGL((k_runs<float, true, false>), g, b, sm, s, a);
         else hipLaunchKernelGGL((k_runs<float, false, false>), g, b, sm, s, a);
     }
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int ivr_launch_blockruns(ivr_ctx *ctx, RenderArgs &a)
+{
+    ivr_set_grid(a, ctx, a.W, a.H);
+    hipLaunchKernelGGL(k_blockruns, dim3((unsigned)a.nblk_pad), dim3(IVR_FRAME_BLOCK), 0, ctx->stream, a);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }
