@@ -115,7 +115,10 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
 __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_blockruns(const RenderArgs a)
 {
     int i, j;
-    if (!block_pixel(a, blockIdx.x, i, j)) return; /* uniform per workgroup */
+    if (!block_pixel(a, blockIdx.x, i, j)) { /* padding: uniform per workgroup */
+        if (threadIdx.x == 0) a.blockmax[blockIdx.x] = 0;
+        return;
+    }
     int cnt = 0;
     if (i < a.H && j < a.W) {
         float d[3];
