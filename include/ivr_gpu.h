@@ -244,8 +244,9 @@ int ivr_batcher_destroy(ivr_batcher *b);
 /* make a resident volume servable; returns its index (>= 0) or an error */
 int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol);
 /* make a trained surrogate servable: requests with the returned index are
- * rendered from the network (one launch per request on the batch's stream,
- * beside the batch kernel), bit-identical to ivr_surrogate_render.  The
+ * rendered from the network (the surrogate requests of a batch share one
+ * launch on its stream, beside the batch kernel), bit-identical to
+ * ivr_surrogate_render.  The
  * per-brick fallback flags are copied now; vol (kept alive by the caller,
  * and charged to the budget only when it is registered itself) must be
  * given when a flag is set.  No counterpart in the reference. */
