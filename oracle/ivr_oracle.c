@@ -7,18 +7,21 @@
  *
  * A clean-room restatement of the reference render path: ray set-up
  * (mathutil.c:1115-1154, main.c:937-1009), slab test (ray.c:276-318),
- * sampling (ray.c:87-181, 239-273), transfer functions (transferfuncs.c),
+ * sampling (ray.c:87-273), the voxel walk (ray.c:333-483), the gradient
+ * field (mathutil.c:242-338), transfer functions (transferfuncs.c),
  * colormaps (colormap.c), tile compositing (main.c:1203-1218) and the RGBA8
  * quantiser (main.c:1404).  Compat mode reproduces the reference's float
  * operation order exactly (this file is compiled with -ffp-contract=off, so
  * fmaf() is the only fused multiply-add and compat mode uses none); config
  * mode is the benchmark contract (see ivr_oracle.h).
  *
- * Pinning (tests/test_oracle.py): compat mode, samplers, slab test, transfer
- * functions, colormaps and tile compositing are compared bit-for-bit with the
- * reference compiled from its own sources (oracle/_ref, golden fixtures under
+ * Pinning (tests/test_oracle.py): compat mode, samplers (linear and
+ * nearest), slab test, sample marching, the voxel walk, transfer functions,
+ * colormaps and tile compositing are compared bit-for-bit with the reference
+ * compiled from its own sources (oracle/_ref, golden fixtures under
  * tests/golden); config mode against closed forms (affine fields under true
- * trilinear, constant-slab opacity, ERT bound, BTF==FTB) and the FP64 path.
+ * trilinear, constant-slab opacity, ERT bound, BTF==FTB) and the FP64 path;
+ * the gradient field against numpy.gradient and an affine field.
  */
 #include "ivr_oracle.h"
 
