@@ -509,6 +509,30 @@ def test_gradient_field_matches_oracle(ctx, dims, f16, brick):
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
 
 
+def test_gradient_field_into_device_memory(ctx):
+    """the stream-ordered device-destination form of ivr_volume_gradient
+    writes the same field as the host form"""
+    import ctypes as C
+    nx, ny, nz = 70, 41, 33
+    host = np.random.default_rng(3).uniform(-1.0, 1.0, (nz, ny, nx)).astype(np.float32)
+    vol = ctx.volume(nx, ny, nz, ivr.IVR_F32, brick=32)
+    vol.upload(host)
+    ref = vol.gradient()
+    hip = C.CDLL("libamdhip64.so")
+    dst = C.c_void_p()
+    nbytes = nx * ny * nz * 16
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    try:
+        vol.gradient(device_ptr=dst.value)
+        ctx.sync()
+        got = np.empty((nz, ny, nx, 4), np.float32)
+        assert hip.hipMemcpy(C.c_void_p(got.ctypes.data), dst, C.c_size_t(nbytes), 2) == 0  # to host
+    finally:
+        hip.hipFree(dst)
+    vol.close()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
 def test_gradient_field_c2_full_size(ctx):
     """the gradient of the 512^3 generated volume (2x2x2 bricks of 256^3),
     bit-exact with the oracle over every voxel"""
