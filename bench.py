@@ -92,7 +92,7 @@ def main():
         bench_serving(args)
         return
     c = args.config
-    cfg = configs.CONFIGS[c]
+    cfg = configs.WORKLOADS[c]
     W, H = cfg["image"]
     skip = cfg["skip"] if args.skip < 0 else bool(args.skip)
     names = {"fused": ivr.FUSED, "sort_last": ivr.SORT_LAST}
@@ -261,7 +261,7 @@ def bench_surrogate(args):
     import ivr_amd as ivr
     from ivr_amd import configs
     c = args.sg_volume
-    cfg = configs.CONFIGS[c]
+    cfg = configs.WORKLOADS[c]
     ctx = ivr.Context(0)
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(c))
@@ -455,7 +455,7 @@ def bench_gradient(args):
     import ivr_amd as ivr
     from ivr_amd import configs
     import oracle
-    cfg = configs.CONFIGS[2]
+    cfg = configs.WORKLOADS[2]
     nx, ny, nz = cfg["dims"]
     nvox = nx * ny * nz
     ctx = ivr.Context(0)

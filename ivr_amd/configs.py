@@ -11,7 +11,7 @@ ALPHA_MAX = 0.05
 STEP_VOX = 0.5
 ERT = 0.99
 
-CONFIGS = {
+WORKLOADS = {
     1: dict(name="C1 64^3 fp32 blobs -> 256^2", dims=(64, 64, 64), dtype=ivr.IVR_F32, brick=64,
             image=(256, 256), skip=False, composite=ivr.FUSED,
             gen=dict(kind=ivr.GEN_BLOBS, nblobs=8)),
@@ -26,7 +26,7 @@ CONFIGS = {
 
 
 def gen_params(c, dims=None):
-    cfg = CONFIGS[c]
+    cfg = WORKLOADS[c]
     g = dict(cfg["gen"])
     return ivr.gen_params(seed=1000 + c, **g)
 
@@ -36,5 +36,5 @@ def tf_lut(c, jitter=0.0, jitter_seed=0):
 
 
 def state(c, width=None, height=None, seed=None):
-    w, h = CONFIGS[c]["image"]
+    w, h = WORKLOADS[c]["image"]
     return ivr.orbit_state(3000 + c if seed is None else seed, width or w, height or h)

@@ -371,7 +371,7 @@ def test_minimum_sizes(ctx):
 def test_c2_full_size_rows_match_oracle(ctx):
     """C2 at its full size: 512^3 generated on the GPU, 1024^2 frame; a band
     of rows checked against the oracle, determinism of repeated renders"""
-    cfg = configs.CONFIGS[2]
+    cfg = configs.WORKLOADS[2]
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(2))
     info = vol.info()
@@ -409,7 +409,7 @@ def test_c3_full_size_rows_match_oracle(ctx):
     70 % zero), 1920x1080 with macrocell skipping; a band of rows of the
     sort-last and the fused frame bit-exact against the oracle (per-brick
     tiles composited, and one pass), sample counts included"""
-    cfg = configs.CONFIGS[3]
+    cfg = configs.WORKLOADS[3]
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(3))
     assert vol.info()["nbricks"] == (4, 4, 4)
