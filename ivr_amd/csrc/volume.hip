@@ -107,7 +107,7 @@ __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *
  * takes f[n-2] - f[n-3], read here as f[n-1] - f[n-2] like the other faces).
  * w = sqrtf(x*x + y*y + z*z) (mag3f, mathutil.c:233-236).  Output x fastest.
  *
- * A workgroup covers a 64 x 4 (x, y) tile, one wavefront per row, and
+ * A workgroup covers a 64 x 8 (x, y) tile, one wavefront per row, and
  * marches GRAD_ZC slices along z keeping f(z-1), f(z), f(z+1) in registers,
  * so a voxel row is fetched from HBM about 1.5 times (its x / y neighbours
  * mostly come from the caches); the output is written with non-temporal
@@ -118,11 +118,11 @@ __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *
  * storage coordinates run over [0, B+2) while a voxel's are in [1, B+1]. */
 #define GRAD_ZC 16
 template <typename VT>
-__global__ __launch_bounds__(256) void k_gradient(BrickGeom g, const unsigned long long *boff,
+__global__ __launch_bounds__(512) void k_gradient(BrickGeom g, const unsigned long long *boff,
                                                   const VT *src, float4 *out)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int y = blockIdx.y * 8 + (threadIdx.x >> 6);
     const int z0 = blockIdx.z * GRAD_ZC;
     if (x >= g.n[0] || y >= g.n[1]) return;
     const int z1 = min(z0 + GRAD_ZC, g.n[2]);
@@ -492,13 +492,13 @@ int ivr_volume_gradient(ivr_ctx *ctx, const ivr_volume *v, float *dst, int dst_o
         out = (float4 *)staging.p;
     }
     const BrickGeom g = geom_of(v);
-    const dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 3) / 4),
+    const dim3 grid((unsigned)((g.n[0] + 63) / 64), (unsigned)((g.n[1] + 7) / 8),
                     (unsigned)((g.n[2] + GRAD_ZC - 1) / GRAD_ZC));
     if (v->dtype == IVR_F16)
-        hipLaunchKernelGGL(k_gradient<h16>, grid, dim3(256), 0, ctx->stream, g, v->d_boff,
+        hipLaunchKernelGGL(k_gradient<h16>, grid, dim3(512), 0, ctx->stream, g, v->d_boff,
                            (const h16 *)v->data, out);
     else
-        hipLaunchKernelGGL(k_gradient<float>, grid, dim3(256), 0, ctx->stream, g, v->d_boff,
+        hipLaunchKernelGGL(k_gradient<float>, grid, dim3(512), 0, ctx->stream, g, v->d_boff,
                            (const float *)v->data, out);
     IVR_HIP(hipGetLastError());
     if (!dst_on_device) {
