@@ -457,7 +457,7 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
         a.runcount = counts ? f->d_runcount : nullptr;
         if (counts) {
             /* the workgroups of runs that no ray of their pixel block has are skipped
-             * (C3 1.22 -> 1.18 ms per frame); the count pass is not part of render_ms */
+             * (C3 1.22 -> 1.18 ms per frame); the bound pass is not part of render_ms */
             ivr_set_grid(a, ctx, f->W, f->H);
             if ((size_t)a.nblk_pad > f->blockmax_cap) {
                 ivr_dfree(ctx, f->d_blockmax, f->blockmax_cap * sizeof(int));

@@ -110,8 +110,10 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_runs(const RenderArgs a)
     count_add(a.counters, 2, wrote);
 }
 
-/* the most runs of a ray of each pixel block, counted by the boundary walk
- * k_runs uses (run_end from the first sample until the ray's end) */
+/* an upper bound of the most runs of a ray of each pixel block: each axis
+ * brick index is monotone along a ray, so the bricks its samples visit
+ * number at most 1 + the sum over axes of |last brick - first brick| (fewer
+ * where it crosses an edge or corner); k_runs skips only runs past it */
 __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_blockruns(const RenderArgs a)
 {
     int i, j;
@@ -124,9 +126,12 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_blockruns(const RenderArgs 
         float d[3];
         cfg_dir(a, i, j, d);
         Ray r;
-        if (ray_setup(a, a.o, d, r))
-            for (int k = 0; k < r.n; cnt++)
-                k = run_end(a, r, k, brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k)));
+        if (ray_setup(a, a.o, d, r) && r.n > 0) {
+            const Pos p0 = cfg_pos(a, r.go, r.gd, r.tin, r.dt, 0);
+            const Pos p1 = cfg_pos(a, r.go, r.gd, r.tin, r.dt, r.n - 1);
+            cnt = 1 + abs((p1.j >> a.bshift) - (p0.j >> a.bshift)) + abs((p1.i >> a.bshift) - (p0.i >> a.bshift)) +
+                  abs((p1.k >> a.bshift) - (p0.k >> a.bshift));
+        }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cnt = max(cnt, __shfl_xor(cnt, off, 64));
