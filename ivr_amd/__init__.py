@@ -349,8 +349,14 @@ class Frame:
         check(lib.ivr_frame_create(ctx.h, width, height, int(keep_rgba32), C.byref(h)))
         self.h = h
 
-    def download(self, rgba32=False):
-        img8 = np.empty((self.height, self.width, 4), np.uint8)
+    def download(self, rgba32=False, out=None):
+        """RGBA8 image (row 0 = bottom) into `out` (a C-contiguous uint8 array of the
+        frame's shape, reused across frames) or a new array; with rgba32 also RGBA32F"""
+        shape = (self.height, self.width, 4)
+        if out is not None and (out.shape != shape or out.dtype != np.uint8
+                                or not out.flags["C_CONTIGUOUS"] or not out.flags["WRITEABLE"]):
+            raise ValueError("out must be a writeable C-contiguous uint8 array of shape %s" % (shape,))
+        img8 = np.empty(shape, np.uint8) if out is None else out
         img32 = np.empty((self.height, self.width, 4), np.float32) if rgba32 else None
         check(lib.ivr_frame_download(self.ctx.h, self.h, _ptr(img8),
                                      _ptr(img32) if rgba32 else None))

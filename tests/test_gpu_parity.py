@@ -168,6 +168,17 @@ def test_config_c1_matches_oracle(ctx, c1_host):
     assert m <= 1
     assert stats["samples"] == s_ref
     assert same == 1.0
+    # the download into a caller's reused buffer is the same image
+    tf = ctx.tf(lut)
+    fr = ctx.frame(256, 256)
+    ctx.render(vol, tf, d, fr)
+    buf = np.full((256, 256, 4), 7, np.uint8)
+    assert fr.download(out=buf) is buf
+    assert np.array_equal(buf, img8)
+    with pytest.raises(ValueError):
+        fr.download(out=np.empty((256, 255, 4), np.uint8))
+    fr.close()
+    tf.close()
     vol.close()
 
 
