@@ -18,10 +18,12 @@ bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE for the same
 command, scaled by the counters' reading for a copy of known size
 (profiles/traffic.json).
 cpu_baseline: the oracle (the CPU restatement of the same workload, same
-samples and arithmetic) on the host cores over a band of rows.
+samples and arithmetic, one pass per ray) on the host cores (the CPUs of the
+process's affinity mask, capped by OMP_NUM_THREADS), whole frames.
 reference_cpu: the reference ray.c renderer compiled from its sources
-(oracle/_ref), single core, against this library's compat mode on the GPU
-over the same volume, camera and LUT (bit-identical images).
+(oracle/_ref) with its own OpenMP loop on the same thread count and on one
+core, against this library's compat mode on the GPU over the same volume,
+camera and LUT (bit-identical images).
 
 --config 4 (learned surrogate of the C2 volume), --config 5 (steering
 service) and --gradient (the central-difference gradient field of the C2
@@ -78,6 +80,8 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    # every OpenMP CPU leg (the oracle, the reference) runs on the same thread count
+    os.environ["OMP_NUM_THREADS"] = str(cpu_threads())
     import numpy as np
     import ivr_amd as ivr
     from ivr_amd import configs
@@ -363,7 +367,7 @@ def cpu_surrogate_baseline(sg, target_s):
     import numpy as np
     import oracle
     mlp, table = sg.get(brick=0, table=True)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     rng = np.random.default_rng(4004)
     n = 1 << 12
     t = time.perf_counter()
@@ -376,6 +380,7 @@ def cpu_surrogate_baseline(sg, target_s):
     dt = time.perf_counter() - t
     log("cpu surrogate baseline: %d points in %.2fs on %d threads" % (n, dt, threads))
     return {"value": round(n / dt / 1e6, 3), "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "host": host_info(),
             "sample": "%d uniform points of brick 0 through the oracle's encoding + MLP (fp64, bf16 "
                       "rounding), one timed run" % n}
 
@@ -496,7 +501,7 @@ def bench_gradient(args):
     vsize = 2 if cfg["dtype"] == ivr.IVR_F16 else 4
     bytes_alg = nvox * (vsize + 16)
     achieved = bytes_alg / (ms * 1e-3)
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = cpu_threads()
     out = {
         "metric": "Gvoxels/s, gradient field 512^3 fp32", "value": round(nvox / ms / 1e6, 3),
         "unit": "Gvoxels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -508,7 +513,7 @@ def bench_gradient(args):
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": measured_traffic("gradient field of " + cfg["name"], "gradient", False)},
         "cpu_baseline": {"value": round(nvox / cpu_s / 1e9, 3), "unit": "Gvoxels/s", "cores": threads,
-                         "kind": "port", "sample": "the whole %dx%dx%d volume, one timed run" % (nx, ny, nz)},
+                         "kind": "port", "host": host_info(), "sample": "the whole %dx%dx%d volume, one timed run" % (nx, ny, nz)},
     }
     print(json.dumps(out), flush=True)
     vol.close()
@@ -544,33 +549,62 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}))
 
 
+def host_info():
+    """the host the CPU legs ran on: CPU model, CPUs in this process's
+    affinity mask, OMP_NUM_THREADS"""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """threads of the CPU legs: the CPUs this process may run on, capped by
+    OMP_NUM_THREADS where the environment sets it (the box's CPU share)"""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
 def _oracle_frame(oracle, host, st, lut, comp, brick, rows, threads, f64=False):
     """the oracle's render of rows [r0, r1) of the same workload: one pass
-    (fused) or per-brick tiles in visibility order + compositetiles"""
-    import ctypes as C
+    per ray, fused or sort-last (per-brick runs composited far to near); the
+    FP64 sort-last form is the per-brick tiles composited in visibility order"""
     import numpy as np
     import ivr_amd as ivr
     v = oracle.view_of(st)
     if comp == ivr.FUSED:
         return oracle.render(host, v, lut, oracle.params(), rows=rows, nthreads=threads, f64=f64)
+    if not f64:
+        return oracle.render(host, v, lut, oracle.params(brick=brick, sort_last=True), rows=rows,
+                             nthreads=threads)
+    import ctypes as C
     cam = np.zeros(23, np.float32)
     rect = (C.c_int * 4)()
     bmin, bmax = oracle.box(host.shape[2], host.shape[1], host.shape[0])
-    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    fp = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
     oracle.lib.ivro_camera(fp(bmin), fp(bmax), C.byref(v), fp(cam), rect)
     dims = (host.shape[2], host.shape[1], host.shape[0])
     nb = [(dims[r] - 2) // brick + 1 for r in range(3)]
     eb = [int(np.floor((cam[r] - bmin[r]) * (dims[r] / (bmax[r] - bmin[r])) / brick)) for r in range(3)]
     ids = sorted([(x, y, z) for z in range(nb[2]) for y in range(nb[1]) for x in range(nb[0])],
                  key=lambda b: (sum(abs(b[r] - eb[r]) for r in range(3)), b[0] + nb[0] * (b[1] + nb[1] * b[2])))
-    tiles, s = [], 0
+    d, s = None, 0
+    tiles = []
     for b in ids:
         t, k = oracle.render(host, v, lut, oracle.params(brick=brick, brick_id=b), rows=rows, nthreads=threads,
-                             f64=f64)
+                             f64=True)
         tiles.append(t)
         s += k
-    if not f64:
-        return oracle.composite_tiles(np.stack(tiles)), s
     d = np.zeros_like(tiles[0])  # compositetiles in double: far to near, d = s + (1 - s.a) d
     for t in reversed(tiles):
         d = t + (1.0 - t[..., 3:4]) * d
@@ -578,25 +612,33 @@ def _oracle_frame(oracle, host, st, lut, comp, brick, rows, threads, f64=False):
 
 
 def cpu_baseline(vol, st, lut, gpu_img, W, H, comp, brick, target_s):
-    """The oracle on a band of rows through the image centre, sized to about
-    target_s seconds; median of up to 5 timed runs after a warm-up."""
+    """The oracle (the CPU restatement of the same workload: same samples,
+    same float operations, one pass per ray) on the host cores.  A timed run
+    renders the whole frame `reps` times, or a band of rows through the image
+    centre when one frame takes longer than target_s / 5; median of 5 timed
+    runs after a warm-up."""
     import numpy as np
     import oracle
     host = vol.download()
-    threads = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     mid = H // 2
     t = time.perf_counter()
     _oracle_frame(oracle, host, st, lut, comp, brick, (mid - 4, mid + 4), threads)
-    dt = time.perf_counter() - t
-    rows = int(min(H, max(8, 8 * target_s / max(dt, 1e-3))))
-    r0, r1 = max(0, mid - rows // 2), min(H, mid - rows // 2 + rows)
-    log("cpu baseline: %d threads, rows %d..%d" % (threads, r0, r1))
+    frame_s = (time.perf_counter() - t) * H / 8.0
+    per_run = target_s / 5.0
+    if frame_s <= per_run:
+        r0, r1, reps = 0, H, max(1, int(per_run / max(frame_s, 1e-4)))
+    else:
+        rows = int(min(H, max(8, H * per_run / frame_s)))
+        r0, r1, reps = max(0, mid - rows // 2), min(H, mid - rows // 2 + rows), 1
+    log("cpu baseline: %d threads, rows %d..%d, %d frame(s) per timed run" % (threads, r0, r1, reps))
     _oracle_frame(oracle, host, st, lut, comp, brick, (r0, r1), threads)  # warm-up
     times = []
-    while len(times) < 5 and sum(times) < target_s:
+    for _ in range(5):
         t = time.perf_counter()
-        img32, n = _oracle_frame(oracle, host, st, lut, comp, brick, (r0, r1), threads)
-        times.append(time.perf_counter() - t)
+        for _ in range(reps):
+            img32, n = _oracle_frame(oracle, host, st, lut, comp, brick, (r0, r1), threads)
+        times.append((time.perf_counter() - t) / reps)
     dt = float(np.median(times))
     ref8 = oracle.quantize(img32[r0:r1])
     d = int(np.abs(ref8.astype(int) - gpu_img[r0:r1].astype(int)).max())
@@ -604,16 +646,19 @@ def cpu_baseline(vol, st, lut, gpu_img, W, H, comp, brick, target_s):
     f0, f1 = mid - 8, mid + 8
     i64, _ = _oracle_frame(oracle, host, st, lut, comp, brick, (f0, f1), threads, f64=True)
     d64 = int(np.abs(oracle.quantize(i64[f0:f1]).astype(int) - gpu_img[f0:f1].astype(int)).max())
-    log("cpu baseline: %.2fs for %d samples; max |gpu-cpu| on those rows = %d/255, vs fp64 %d/255"
+    log("cpu baseline: %.3fs per pass over %d samples; max |gpu-cpu| on those rows = %d/255, vs fp64 %d/255"
         % (dt, n, d, d64))
     return {
         "value": round(n / dt / 1e9, 4),
         "unit": "Gsamples/s",
         "cores": threads,
         "kind": "port",
-        "sample": "rows %d-%d of the %dx%d frame, %d samples, median of %d timed runs (%.2fs each) after "
-                  "1 warm-up; max |gpu-cpu| %d/255; max |gpu-fp64 oracle| %d/255 on rows %d-%d"
-                  % (r0, r1, W, H, n, len(times), dt, d, d64, f0, f1),
+        "host": host_info(),
+        "build": "gcc -O3 -march=x86-64-v3 -ffp-contract=off -fopenmp, OpenMP static schedule over rows",
+        "sample": "rows %d-%d of the %dx%d frame, %d samples, %s, %d frame(s) per timed run, median of 5 runs "
+                  "(%.3fs per frame) after 1 warm-up; max |gpu-cpu| %d/255; max |gpu-fp64 oracle| %d/255 on rows "
+                  "%d-%d" % (r0, r1, W, H, n, "one pass per ray" if comp == 0 else
+                             "one pass per ray, per-brick runs combined far to near", reps, dt, d, d64, f0, f1),
     }
 
 
@@ -633,8 +678,9 @@ class stdout_to_stderr:
 def reference_cpu(ctx, vol, st, lut, W, H):
     """The reference's own ray.c/transferfuncs.c (compiled from its sources)
     rendering one frame of the same volume (values x255), camera and LUT with
-    CHEM, single-threaded as the reference is; against the GPU compat mode
-    on identical input, whose image must be bit-identical."""
+    CHEM, its OpenMP loop on the CPU legs' thread count and on one core;
+    against the GPU compat mode on identical input, whose image must be
+    bit-identical."""
     import ctypes as C
     import numpy as np
     import ivr_amd as ivr
@@ -691,20 +737,29 @@ def reference_cpu(ctx, vol, st, lut, W, H):
     nc = np.array(st.newconst, np.float32)
     out = np.zeros((H, W, 4), np.float32)
     rect = (C.c_int * 4)()
-    with stdout_to_stderr():
-        t = time.perf_counter()
-        ref.ivrref_render(f(mvm), f(pm), f(vv), f(nc), C.c_float(st.alpha), W, H, f(out), rect)
-        c_s = time.perf_counter() - t
-    same = bool(np.array_equal(out.view(np.uint32), g32.view(np.uint32)))
-    log("reference ray.c: %.2fs for %d samples (1 core); GPU compat %.3f ms; images identical: %s"
-        % (c_s, S, g_ms, same))
+    threads = cpu_threads()
+    timed = {}
+    for nt in (threads, 1):  # the reference's OpenMP pixel loop (main.c:983) on the box's share, then one core
+        used = ref.ivrref_set_threads(nt)
+        with stdout_to_stderr():
+            t = time.perf_counter()
+            ref.ivrref_render(f(mvm), f(pm), f(vv), f(nc), C.c_float(st.alpha), W, H, f(out), rect)
+            timed[used] = time.perf_counter() - t
+        if nt == threads:
+            same = bool(np.array_equal(out.view(np.uint32), g32.view(np.uint32)))
+    c_s = timed[max(timed)]
+    log("reference ray.c: %s for %d samples; GPU compat %.3f ms; images identical: %s"
+        % (", ".join("%.2fs on %d threads" % (v, k) for k, v in sorted(timed.items())), S, g_ms, same))
     return {
         "value": round(S / c_s / 1e9, 5),
         "unit": "Gsamples/s",
-        "cores": 1,
+        "cores": max(timed),
         "kind": "reference",
+        "host": host_info(),
+        "one_core_gsamples_s": round(S / timed[min(timed)] / 1e9, 5) if min(timed) == 1 else None,
         "sample": "one %dx%d frame of the reference compat path (ray.c: 256-sample cap, 0.001 step, CHEM "
-                  "back-to-front) over the same volume x255, camera and LUT; %d samples" % (W, H, S),
+                  "back-to-front; its own OpenMP loop over rows, built as its makefile does) over the same "
+                  "volume x255, camera and LUT; %d samples" % (W, H, S),
         "gpu_same_path_gsamples_s": round(S / g_ms / 1e6, 3),
         "gpu_same_path_ms": round(g_ms, 4),
         "gpu_over_reference": round((S / g_ms / 1e6) / (S / c_s / 1e9), 1),

@@ -386,6 +386,13 @@ int admit(ivr_batcher *b)
     return 0;
 }
 
+/* enqueue the batch of slots `ready` on bt's stream (see launch) */
+int launch_slots(ivr_batcher *b, Batch *bt, const std::vector<int> &ready);
+
+/* Form the next batch from the admitted, unlaunched slots and enqueue it.
+ * When enqueueing fails part way, the batch's stream is drained, each of its
+ * requests completes with the error status and its slot is freed, so no
+ * client waits for a frame that will never come; the batcher stays usable. */
 int launch(ivr_batcher *b)
 {
     std::vector<int> ready;
@@ -401,6 +408,33 @@ int launch(ivr_batcher *b)
     b->next_batch = (bt->stream + 1) & 1;
     /* FIFO order of admission */
     std::sort(ready.begin(), ready.end(), [&](int x, int y) { return b->slots[x].t_admit < b->slots[y].t_admit; });
+    const int rc = launch_slots(b, bt, ready);
+    if (!rc) return 0;
+    (void)hipStreamSynchronize(b->streams[bt->stream]); /* nothing of it may still write a slot */
+    (void)hipGetLastError();
+    const double t = now_s();
+    for (int si : ready) {
+        Slot &s = b->slots[si];
+        ivr_completion c = {};
+        c.id = s.req.id;
+        c.status = rc;
+        c.t_submit = s.req.t_submit;
+        c.t_admit = s.t_admit;
+        c.t_done = t;
+        b->done.push_back(c);
+        b->used -= s.charge;
+        s.busy = false;
+        s.launched = false;
+        b->st.rejected++;
+    }
+    bt->slots.clear();
+    bt->active = false;
+    b->st.bytes_in_use = b->used;
+    return 0;
+}
+
+int launch_slots(ivr_batcher *b, Batch *bt, const std::vector<int> &ready)
+{
     hipStream_t st = b->streams[bt->stream];
     int total = 0, nplain = 0, nsg = 0, sgtotal = 0;
     size_t sglds = 0;

@@ -314,16 +314,38 @@ static void consume(client *c, size_t len)
     c->have -= len;
 }
 
+/* submit the complete messages buffered in c->in while the client has no
+ * frame in flight (a client may send its next STATE, or its finish, before
+ * the reply to the last one); -1 on finish or a malformed message */
+static int parse_buffered(ivr_batcher *bt, client *c)
+{
+    while (!c->busy) {
+        size_t len = message_len(c);
+        if (len == (size_t)-1) return -1;
+        if (!len) break;
+        int rc = handle(bt, c, len);
+        consume(c, len);
+        if (rc) return -1; /* finish or malformed */
+    }
+    return 0;
+}
+
+/* retire finished frames, reply to their clients and start their buffered
+ * requests; -1 when the batcher reports a device error */
 static int pump(ivr_batcher *bt)
 {
     ivr_completion done[64];
-    ivr_batcher_progress(bt, 0);
+    if (ivr_batcher_progress(bt, 0) < 0) {
+        fprintf(stderr, "ivrserver: batcher: %s\n", ivr_last_error());
+        return -1;
+    }
     int n = ivr_batcher_poll(bt, done, 64);
     for (int k = 0; k < n; k++)
         for (int i = 0; i < MAX_CLIENTS; i++)
             if (g_cl[i].used && g_cl[i].busy && g_cl[i].rid == done[k].id) {
                 g_cl[i].busy = 0;
-                if (g_cl[i].zombie || reply(&g_cl[i], done[k].status)) drop_client(&g_cl[i]);
+                if (g_cl[i].zombie || reply(&g_cl[i], done[k].status) || parse_buffered(bt, &g_cl[i]))
+                    drop_client(&g_cl[i]);
             }
     return n;
 }
@@ -340,15 +362,7 @@ static int read_client(ivr_batcher *bt, client *c)
     ssize_t r = recv(c->fd, c->in + c->have, c->cap - c->have, 0);
     if (r <= 0) return -1; /* peer closed mid-STATE or error: drop it */
     c->have += (size_t)r;
-    while (!c->busy) {
-        size_t len = message_len(c);
-        if (len == (size_t)-1) return -1;
-        if (!len) break;
-        int rc = handle(bt, c, len);
-        consume(c, len);
-        if (rc) return -1; /* finish or malformed */
-    }
-    return 0;
+    return parse_buffered(bt, c);
 }
 
 static void usage(void)
@@ -456,7 +470,7 @@ int main(int argc, char **argv)
                 drop_client(&g_cl[map[k]]);
             }
         }
-        pump(bt);
+        if (pump(bt) < 0) break;
         if (work) usleep(100);
     }
     ivr_batcher_destroy(bt);

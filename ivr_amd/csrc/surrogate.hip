@@ -1215,22 +1215,34 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     int hdr[8];
     bool ok = fread(magic, 8, 1, f) == 1 && (!memcmp(magic, "IVRSG001", 8) || !memcmp(magic, "IVRSG002", 8)) &&
               fread(&sp, sizeof sp, 1, f) == 1 && fread(hdr, sizeof hdr, 1, f) == 1;
-    const int nsec = ok && magic[7] == '2' ? 7 : 6;
-    if (ok && (hdr[0] != s->nbricks || hdr[1] != s->B || hdr[2] != s->n[0] || hdr[3] != s->n[1] ||
-               hdr[4] != s->n[2] || hdr[6] != (int)s->T || hdr[7] != SG_NPARAM)) {
+    if (!ok) {
         fclose(f);
-        return ivr_fail(IVR_ERR_ARG, "checkpoint %s does not match this surrogate", path);
+        return ivr_fail(IVR_ERR_IO, "%s is not a surrogate checkpoint", path);
     }
-    std::vector<float> buf;
+    const int nsec = magic[7] == '2' ? 7 : 6;
+    /* the table layout (level resolutions, dense/hashed split, slots) is
+     * fixed at create by log2_table, base_res and growth: a checkpoint made
+     * with other values would load without error and render wrong values */
+    if (hdr[0] != s->nbricks || hdr[1] != s->B || hdr[2] != s->n[0] || hdr[3] != s->n[1] || hdr[4] != s->n[2] ||
+        hdr[6] != (int)s->T || hdr[7] != SG_NPARAM || sp.log2_table != s->sp.log2_table ||
+        sp.base_res != s->sp.base_res || sp.growth != s->sp.growth) {
+        fclose(f);
+        return ivr_fail(IVR_ERR_ARG, "checkpoint %s does not match this surrogate (bricks, dims, table size or "
+                                     "level resolutions differ)", path);
+    }
+    /* every section is read into host memory before the device state is touched,
+     * so a short or failed read leaves the surrogate as it was */
     float *dev[7] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt, (float *)s->d_cellrange};
     const size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, 2 * s->ncells};
+    std::vector<std::vector<float>> buf(nsec);
     for (int k = 0; k < nsec && ok; k++) {
-        buf.resize(cnt[k]);
-        ok = fread(buf.data(), 4, cnt[k], f) == cnt[k] &&
-             hipMemcpy(dev[k], buf.data(), cnt[k] * 4, hipMemcpyHostToDevice) == hipSuccess;
+        buf[k].resize(cnt[k]);
+        ok = fread(buf[k].data(), 4, cnt[k], f) == cnt[k];
     }
     fclose(f);
     if (!ok) return ivr_fail(IVR_ERR_IO, "reading %s failed", path);
+    for (int k = 0; k < nsec; k++)
+        IVR_HIP(hipMemcpy(dev[k], buf[k].data(), cnt[k] * 4, hipMemcpyHostToDevice));
     s->sp = sp;
     s->step = hdr[5];
     int rc = refresh_frags(ctx, s);

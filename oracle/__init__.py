@@ -25,7 +25,7 @@ class View(C.Structure):
 class Params(C.Structure):
     _fields_ = [("mode", C.c_int), ("xfer", C.c_int), ("step_vox", C.c_float), ("ert", C.c_float),
                 ("lutscale", C.c_float), ("lutoffset", C.c_int), ("maxlutindx", C.c_int),
-                ("brick", C.c_int), ("brick_id", C.c_int * 3)]
+                ("brick", C.c_int), ("brick_id", C.c_int * 3), ("sort_last", C.c_int)]
 
 
 def _f(a):
@@ -66,9 +66,9 @@ def view_of(state):
 
 
 def params(mode=CONFIG, xfer=3, step_vox=0.5, ert=0.99, lutscale=255.0, lutoffset=0,
-           maxlutindx=255, brick=0, brick_id=(0, 0, 0)):
+           maxlutindx=255, brick=0, brick_id=(0, 0, 0), sort_last=False):
     p = Params(mode=mode, xfer=xfer, step_vox=step_vox, ert=ert, lutscale=lutscale,
-               lutoffset=lutoffset, maxlutindx=maxlutindx, brick=brick)
+               lutoffset=lutoffset, maxlutindx=maxlutindx, brick=brick, sort_last=int(sort_last))
     p.brick_id[:] = list(brick_id)
     return p
 

@@ -815,6 +815,81 @@ static col4 ray_config(const rctx *c, const float o[3], const float d[3], long l
     return acc;
 }
 
+/* Sort-last in one pass per ray: the samples a brick owns (floor index in
+ * the brick, as in tile mode) form one contiguous run of k, since every axis
+ * index is monotone in k, and the runs come in visibility order.  Each run is
+ * composited from zero with ERT inside it (the remaining samples of a
+ * saturated run are passed over without a fetch), and the partials are
+ * combined far to near with compositetiles' d = s + (1 - s.a) d
+ * (main.c:1206-1215): the same image as the per-brick tiles composited in
+ * visibility order, where the bricks a ray misses add exact zeros. */
+#define IVRO_MAX_RUNS 64
+static col4 ray_sort_last(const rctx *c, const float o[3], const float d[3], long long *ns)
+{
+    col4 out = {0.0f, 0.0f, 0.0f, 0.0f};
+    float tin, tout;
+    if (!slab(c->dconst, o, d, &tin, &tout)) return out;
+    tin = fmaxf(tin, 0.0f);
+    const ivro_params *p = c->p;
+    float vox = (c->bmax[0] - c->bmin[0]) / (float)c->nx;
+    float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    float dt = (vox * p->step_vox) / dlen;
+    int n = (int)(1.0f + (tout - tin) / dt + 0.5f);
+    float sx = (float)c->nx / (c->bmax[0] - c->bmin[0]);
+    float sy = (float)c->ny / (c->bmax[1] - c->bmin[1]);
+    float sz = (float)c->nz / (c->bmax[2] - c->bmin[2]);
+    const float gox = (o[0] - c->bmin[0]) * sx, goy = (o[1] - c->bmin[1]) * sy, goz = (o[2] - c->bmin[2]) * sz;
+    const float gdx = d[0] * sx, gdy = d[1] * sy, gdz = d[2] * sz;
+    const int nx = c->nx, ny = c->ny, nz = c->nz, B = p->brick;
+    col4 part[IVRO_MAX_RUNS];
+    int nrun = 0, cur = -1, done = 0;
+    col4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    long long cnt = 0;
+    for (int k = 0; k < n; k++) {
+        float t = fmaf((float)k, dt, tin);
+        float gx = fmaf(gdx, t, gox);
+        float gy = fmaf(gdy, t, goy);
+        float gz = fmaf(gdz, t, goz);
+        float cx = fminf(fmaxf(gx, 0.0f), (float)(nx - 1));
+        float cy = fminf(fmaxf(gy, 0.0f), (float)(ny - 1));
+        float cz = fminf(fmaxf(gz, 0.0f), (float)(nz - 1));
+        int j = (int)cx, i = (int)cy, kk = (int)cz;
+        j = j > nx - 2 ? nx - 2 : j;
+        i = i > ny - 2 ? ny - 2 : i;
+        kk = kk > nz - 2 ? nz - 2 : kk;
+        const int b = j / B + (nx / B + 1) * (i / B + (ny / B + 1) * (kk / B));
+        if (b != cur) { /* first sample of the next run */
+            if (cur >= 0 && nrun < IVRO_MAX_RUNS) part[nrun++] = acc;
+            acc.r = acc.g = acc.b = acc.a = 0.0f;
+            cur = b;
+            done = 0;
+        }
+        if (done) continue;
+        float s = tri_fetch(c->vol, nx, ny, nz, gx, gy, gz);
+        int idx = (int)(s * p->lutscale) - p->lutoffset;
+        idx = idx < 0 ? 0 : (idx > c->lutsize - 1 ? c->lutsize - 1 : idx);
+        const col4 *e = &c->lut[idx];
+        float a = e->a * c->alpha;
+        float w = (1.0f - acc.a) * a;
+        acc.r = fmaf(w, e->r, acc.r);
+        acc.g = fmaf(w, e->g, acc.g);
+        acc.b = fmaf(w, e->b, acc.b);
+        acc.a = acc.a + w;
+        cnt++;
+        if (acc.a >= p->ert) done = 1;
+    }
+    if (cur >= 0 && nrun < IVRO_MAX_RUNS) part[nrun++] = acc;
+    for (int q = nrun - 1; q >= 0; q--) {
+        const col4 *s = &part[q];
+        out.b = s->b + (1.0f - s->a) * out.b;
+        out.g = s->g + (1.0f - s->a) * out.g;
+        out.r = s->r + (1.0f - s->a) * out.r;
+        out.a = s->a + (1.0f - s->a) * out.a;
+    }
+    *ns += cnt;
+    return out;
+}
+
 static void setup_ctx(rctx *c, const float *vol, int nx, int ny, int nz, const ivro_view *v,
                       const float *lut, int lutsize, const ivro_params *p)
 {
@@ -878,7 +953,7 @@ long long ivro_render(const float *vol, int nx, int ny, int nz, const ivro_view 
             float d[3];
             for (int r = 0; r < 3; r++) d[r] = fmaf(M[r][0], uu, fmaf(M[r][1], vv, m5[r]));
             long long ns = 0;
-            col4 col = ray_config(&c, cam, d, &ns);
+            col4 col = (p->brick > 0 && p->sort_last) ? ray_sort_last(&c, cam, d, &ns) : ray_config(&c, cam, d, &ns);
             total += ns;
             float *px = rgba + ((size_t)i * W + j) * 4;
             px[0] = col.r; px[1] = col.g; px[2] = col.b; px[3] = col.a;

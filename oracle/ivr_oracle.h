@@ -45,6 +45,10 @@ typedef struct {
     int maxlutindx;    /* compat SEISMIC2 upper clamp                        */
     int brick;         /* config tile mode: brick edge in voxels (0 = off)   */
     int brick_id[3];   /* config tile mode: brick whose samples are composited */
+    int sort_last;     /* config with brick > 0: one pass per ray over all bricks,
+                          each brick's partial composited from zero with ERT
+                          inside it, partials combined far to near as
+                          compositetiles does (brick_id unused) */
 } ivro_params;
 
 /* World box of an nx*ny*nz volume (main.c:509-598, utils.c:332-388). */

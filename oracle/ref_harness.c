@@ -14,6 +14,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <omp.h>
 
 #include "types.h"
 #include "connection.h"
@@ -60,6 +61,14 @@ int ivrref_frame_on_gpu(int on)
 {
     g_frame_gpu = on && ivr_compat_createlocalimage != NULL;
     return g_frame_gpu;
+}
+
+/* OpenMP threads of the reference's parallel loops (createlocalimage,
+ * main.c:983) for the following calls; returns the count in effect */
+int ivrref_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 /* init(): volume file of nx*ny*nz little-endian float32, x fastest. */

@@ -245,3 +245,60 @@ def test_relay_mode_is_a_dropin_render_server(ctx, refs):
     assert srv.wait(timeout=60) == 0
     conn.close()
     ls.close()
+
+
+def test_requests_sent_ahead_are_served(ctx, refs):
+    """a client that sends its next requests before the reply to the first
+    (two STATEs and two extension messages in one sendall, then a finish in
+    the same segment as the last request) gets every frame in order, and the
+    finish buffered behind a request ends the relay session once it is served"""
+    port = free_port()
+    srv = start_server(["--listen", str(port), "--volume", "gen:1:64", "--volume", "gen:2:96", "--max", "256",
+                        "256", "--budget-gb", "1"])
+    try:
+        s = socket.create_connection(("127.0.0.1", port), timeout=60)
+        assert recv_all(s, 13) == b"Hello, world!"
+        s.sendall(b"Hello, server!")
+        sts = [ivr.orbit_state(seed, w, h) for seed, (w, h) in zip((3301, 3302), ((120, 88), (96, 72)))]
+        s.sendall(ivr.state_pack(sts[0]) + ivr.state_pack(sts[1]))
+        for st in sts:
+            w, h = st.winwidth, st.winheight
+            img = np.frombuffer(recv_all(s, w * h * 4), np.uint8).reshape(h, w, 4)
+            assert np.array_equal(img, render_ref(ctx, refs[0], st, configs.tf_lut(1)))
+        lut = ivr.tf_build(2002, 0.05, 0.05, 77)
+        s.sendall(ext_message(11, 1, lut, sts[1]) + ext_message(12, 1, lut, sts[0]))
+        for rid, st in ((11, sts[1]), (12, sts[0])):
+            hdr = recv_all(s, 24)
+            assert hdr[:4] == b"IVRR" and struct.unpack("<Qiii", hdr[4:]) == (rid, 0, st.winwidth, st.winheight)
+            img = np.frombuffer(recv_all(s, st.winwidth * st.winheight * 4), np.uint8)
+            assert np.array_equal(img.reshape(st.winheight, st.winwidth, 4), render_ref(ctx, refs[1], st, lut))
+        s.close()
+    finally:
+        srv.terminate()
+        srv.wait(timeout=30)
+    # relay role: the request and the finish arrive in one segment
+    port = free_port()
+    ls = socket.socket()
+    ls.setsockopt(socket.SOL_SOCKET, socket.SO_REUSEADDR, 1)
+    ls.bind(("127.0.0.1", port))
+    ls.listen(1)
+    srv = start_server(["--relay", "127.0.0.1:%d" % port, "--volume", "gen:1:64", "--max", "256", "256",
+                        "--budget-gb", "1"], until="volume 0 ready")
+    try:
+        ls.settimeout(60)
+        conn, _ = ls.accept()
+        conn.settimeout(60)
+        conn.sendall(b"Hello, world!")
+        assert recv_all(conn, 14) == b"Hello, server!"
+        fin = ivr.State()
+        fin.finish = 1
+        conn.sendall(ivr.state_pack(sts[0]) + ivr.state_pack(fin))
+        w, h = sts[0].winwidth, sts[0].winheight
+        img = np.frombuffer(recv_all(conn, w * h * 4), np.uint8).reshape(h, w, 4)
+        assert np.array_equal(img, render_ref(ctx, refs[0], sts[0], configs.tf_lut(1)))
+        assert srv.wait(timeout=60) == 0
+        conn.close()
+    finally:
+        ls.close()
+        if srv.poll() is None:
+            srv.kill()
