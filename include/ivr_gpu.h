@@ -159,6 +159,23 @@ int ivr_frame_stats_get(ivr_ctx *ctx, ivr_frame *fr, ivr_frame_stats *st);
 int ivr_render(ivr_ctx *ctx, const ivr_volume *vol, const ivr_tf *tf,
                const ivr_frame_desc *desc, ivr_frame *fr);
 
+/* ivr_render into caller device memory, allocating nothing.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_rgba8 (W*H*4 bytes) and d_rgba32 (W*H*16 bytes, or NULL) are device
+ * buffers; W, H come from desc->state.  The frame counters and the sort-last
+ * block maxima, run counts and partial planes live in a static device array
+ * of 72 MiB per GPU that the library holds until the process ends: IVR_FUSED
+ * frames of any size fit, IVR_SORT_LAST frames need
+ * runs*W*H*16 + W*H*4 bytes plus a few KiB (runs = sum of bricks per axis - 2;
+ * 1024x1024 with 2x2x2 bricks fits), larger ones fail with IVR_ERR_ARG, and
+ * IVR_SORT_LAST_TILES is IVR_ERR_UNSUPPORTED.  The counters are cleared by a
+ * kernel, no event is recorded and no statistics are kept.  Calls on one GPU
+ * share the array: calls that overlap on one GPU (other contexts' streams,
+ * other threads) race and must be serialised by the caller. */
+int ivr_render_noalloc(ivr_ctx *ctx, const ivr_volume *vol, const ivr_tf *tf,
+                       const ivr_frame_desc *desc, unsigned char *d_rgba8, float *d_rgba32);
+
 /* ---- learned surrogate: per-brick multiresolution hash grids (16 levels
  * x 2 features) and one MLP 32 -> 64 -> 64 -> 1 on bf16 MFMA, trained on
  * samples of a plain volume and rendered in place of its trilinear fetch.

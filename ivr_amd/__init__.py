@@ -111,6 +111,7 @@ def _load():
         "ivr_frame_download": (C.c_int, [P, P, P, P]),
         "ivr_frame_stats_get": (C.c_int, [P, P, C.POINTER(FrameStats)]),
         "ivr_render": (C.c_int, [P, P, P, C.POINTER(FrameDesc), P]),
+        "ivr_render_noalloc": (C.c_int, [P, P, P, C.POINTER(FrameDesc), P, P]),
         "ivr_desc_config": (None, [C.POINTER(FrameDesc)]),
         "ivr_state_orbit": (None, [C.c_uint64, C.c_int, C.c_int, C.POINTER(State),
                                    C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -266,6 +267,12 @@ class Context:
 
     def render(self, vol, tf, desc, frame):
         check(lib.ivr_render(self.h, vol.h, tf.h, C.byref(desc), frame.h))
+
+    def render_noalloc(self, vol, tf, desc, d_rgba8, d_rgba32=None):
+        """ivr_render_noalloc into caller device buffers (addresses as ints);
+        returns the status code (0 or a negative IVR_ERR_*) without raising"""
+        return lib.ivr_render_noalloc(self.h, vol.h, tf.h, C.byref(desc), C.c_void_p(d_rgba8),
+                                      None if d_rgba32 is None else C.c_void_p(d_rgba32))
 
     def surrogate(self, vol, **kw):
         return Surrogate(self, vol, **kw)

@@ -15,12 +15,40 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ivr_internal.h"
 
+/* Scratch of ivr_render_noalloc: the frame counters, the sort-last block
+ * maxima, run counts and partial planes of a frame rendered into caller
+ * buffers.  One copy per GPU, held while the library is loaded; host code
+ * uses only the device address hipGetSymbolAddress gives for it (the name
+ * itself is a host-side stand-in).  Calls on one GPU share it. */
+#define IVR_SCRATCH_BYTES (72ull << 20)
+static __device__ __align__(256) unsigned char scratch[IVR_SCRATCH_BYTES];
+
 namespace {
 thread_local char g_err[512] = "";
+
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[64] = {}; /* per device */
+
+/* the current device's address of `scratch` (device made current by the caller) */
+int scratch_for(int device, unsigned char **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) return ivr_fail(IVR_ERR_ARG, "device %d", device);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return ivr_fail(IVR_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return IVR_OK;
+}
 }
 
 int ivr_fail(int code, const char *fmt, ...)
@@ -84,6 +112,11 @@ int ivr_ctx_create(int device, ivr_ctx **out)
         c->block_w = (w == 16 || w == 32 || w == 64) ? w : 0;
     }
     if (const char *e = getenv("IVR_SL_SPLIT")) c->sl_split = atoi(e) != 0;
+    int rc = scratch_for(device, &c->scratch);
+    if (rc) {
+        delete c;
+        return rc;
+    }
     IVR_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto &ev : c->ev) IVR_HIP(hipEventCreate(&ev));
     *out = c;
@@ -404,12 +437,29 @@ int render_sort_last(ivr_ctx *ctx, const ivr_volume *v, const ivr_frame_desc *d,
 
 } // namespace
 
-int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *f)
+namespace {
+
+/* device buffer of `need` elements for a frame: grown (freed and allocated)
+ * for a library frame, checked against its capacity for a frame in scratch */
+template <typename T>
+int frame_buffer(ivr_ctx *ctx, bool noalloc, T **p, size_t *cap, size_t need)
 {
-    if (!ctx || !v || !tf || !d || !f) return ivr_fail(IVR_ERR_ARG, "null argument");
-    if (d->state.winwidth != f->W || d->state.winheight != f->H)
+    if (need <= *cap) return IVR_OK;
+    if (noalloc)
+        return ivr_fail(IVR_ERR_ARG, "frame needs %zu bytes of scratch beyond the %llu the library holds", need * sizeof(T),
+                        IVR_SCRATCH_BYTES);
+    ivr_dfree(ctx, *p, *cap * sizeof(T));
+    *cap = 0;
+    int rc = ivr_dalloc(ctx, (void **)p, need * sizeof(T));
+    if (!rc) *cap = need;
+    return rc;
+}
+
+int check_desc(const ivr_volume *v, const ivr_tf *tf, const ivr_frame_desc *d, int W, int H)
+{
+    if (d->state.winwidth != W || d->state.winheight != H)
         return ivr_fail(IVR_ERR_ARG, "state %dx%d does not match frame %dx%d", d->state.winwidth,
-                        d->state.winheight, f->W, f->H);
+                        d->state.winheight, W, H);
     if (d->mode != IVR_MODE_COMPAT && d->mode != IVR_MODE_CONFIG)
         return ivr_fail(IVR_ERR_ARG, "unknown mode %d", d->mode);
     if (d->mode == IVR_MODE_CONFIG && !(d->step_vox > 0.0f))
@@ -421,37 +471,47 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
     if ((size_t)tf->n * sizeof(float4) + (size_t)v->nbricks * 8 > 65536)
         return ivr_fail(IVR_ERR_UNSUPPORTED, "LUT of %d entries and %d bricks exceed 64 KiB of LDS", tf->n,
                         v->nbricks);
+    return IVR_OK;
+}
+
+/* One frame on the context stream into f's buffers.  noalloc: f's buffers
+ * are the caller's image and the scratch array (capacities set by the
+ * caller); nothing is allocated, counters are cleared by a kernel, no event
+ * is recorded. */
+int render_into(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *f,
+                bool noalloc)
+{
     ivr_cam cam;
     ivr_cam_setup(&d->state, v->n[0], v->n[1], v->n[2], &cam);
     RenderArgs a;
     ivr_fill_args(a, cam, v, tf, d, f);
-    IVR_HIP(hipMemsetAsync(f->d_counters, 0, IVR_CTR_BYTES, ctx->stream));
-    f->timed = 1;
-    f->composited = 0;
     int rc;
+    if (noalloc) {
+        if ((rc = ivr_launch_zero_u64(ctx->stream, f->d_counters, IVR_CTR_BYTES / 8))) return rc;
+    } else {
+        IVR_HIP(hipMemsetAsync(f->d_counters, 0, IVR_CTR_BYTES, ctx->stream));
+    }
+    f->timed = !noalloc;
+    f->composited = 0;
     const int skip = d->mode == IVR_MODE_CONFIG && d->skip;
     if (skip && (rc = ivr_launch_cells(ctx, v, a))) return rc;
-    IVR_HIP(hipEventRecord(f->e[0], ctx->stream)); /* render_ms: the render kernel(s) alone */
+    if (f->timed) IVR_HIP(hipEventRecord(f->e[0], ctx->stream)); /* render_ms: the render kernel(s) alone */
     if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST_TILES) {
+        if (noalloc) return ivr_fail(IVR_ERR_UNSUPPORTED, "materialised tiles need a library frame");
         if ((rc = render_sort_last(ctx, v, d, cam, a, f))) return rc;
         IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
         return IVR_OK;
     }
     if (d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST && ctx->sl_split) {
         const int runs = v->nb[0] + v->nb[1] + v->nb[2] - 2;
-        const size_t need = (size_t)runs * f->W * f->H;
-        if (need > f->tiles_cap) {
-            ivr_dfree(ctx, f->d_tiles, f->tiles_cap * sizeof(float4));
-            f->tiles_cap = 0;
-            if ((rc = ivr_dalloc(ctx, (void **)&f->d_tiles, need * sizeof(float4)))) return rc;
-            f->tiles_cap = need;
-        }
         /* Per-ray run counts spare the zero planes of rays that have few of
          * the frame's run slots (4x4x4 bricks: 10 slots, C3 1.38 -> 1.29 ms);
          * with at most 4 slots (2x2x2 bricks) writing the zero planes was
          * measured faster (C2 1.08 vs 1.12 ms). */
         const bool counts = runs > 4;
-        if (counts && !f->d_runcount && (rc = ivr_dalloc(ctx, (void **)&f->d_runcount, (size_t)f->W * f->H * 4)))
+        size_t rc_cap = f->d_runcount ? (size_t)f->W * f->H : 0;
+        if ((rc = frame_buffer(ctx, noalloc, &f->d_tiles, &f->tiles_cap, (size_t)runs * f->W * f->H)) ||
+            (counts && (rc = frame_buffer(ctx, noalloc, &f->d_runcount, &rc_cap, (size_t)f->W * f->H))))
             return rc;
         a.tile = f->d_tiles;
         a.runcount = counts ? f->d_runcount : nullptr;
@@ -459,25 +519,58 @@ int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_fr
             /* the workgroups of runs that no ray of their pixel block has are skipped
              * (C3 1.22 -> 1.18 ms per frame); the bound pass is not part of render_ms */
             ivr_set_grid(a, ctx, f->W, f->H);
-            if ((size_t)a.nblk_pad > f->blockmax_cap) {
-                ivr_dfree(ctx, f->d_blockmax, f->blockmax_cap * sizeof(int));
-                f->blockmax_cap = 0;
-                if ((rc = ivr_dalloc(ctx, (void **)&f->d_blockmax, (size_t)a.nblk_pad * sizeof(int)))) return rc;
-                f->blockmax_cap = (size_t)a.nblk_pad;
-            }
+            if ((rc = frame_buffer(ctx, noalloc, &f->d_blockmax, &f->blockmax_cap, (size_t)a.nblk_pad))) return rc;
             a.blockmax = f->d_blockmax;
             if ((rc = ivr_launch_blockruns(ctx, a))) return rc;
-            IVR_HIP(hipEventRecord(f->e[0], ctx->stream));
+            if (f->timed) IVR_HIP(hipEventRecord(f->e[0], ctx->stream));
         }
         if ((rc = ivr_launch_runs(ctx, v, a, skip))) return rc;
-        IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
+        if (f->timed) IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
         f->composited = 1;
         if ((rc = ivr_launch_combine_runs(ctx, a, runs))) return rc;
-        IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
+        if (f->timed) IVR_HIP(hipEventRecord(f->e[2], ctx->stream));
         return IVR_OK;
     }
     if ((rc = ivr_launch_render(ctx, v, a, skip, d->mode == IVR_MODE_CONFIG && d->composite == IVR_SORT_LAST)))
         return rc;
-    IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
+    if (f->timed) IVR_HIP(hipEventRecord(f->e[1], ctx->stream));
     return IVR_OK;
+}
+
+} // namespace
+
+int ivr_render(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_frame_desc *d, ivr_frame *f)
+{
+    if (!ctx || !v || !tf || !d || !f) return ivr_fail(IVR_ERR_ARG, "null argument");
+    int rc = check_desc(v, tf, d, f->W, f->H);
+    if (rc) return rc;
+    return render_into(ctx, v, tf, d, f, false);
+}
+
+int ivr_render_noalloc(ivr_ctx *ctx, const ivr_volume *v, const ivr_tf *tf, const ivr_frame_desc *d,
+                       unsigned char *d_rgba8, float *d_rgba32)
+{
+    if (!ctx || !v || !tf || !d || !d_rgba8) return ivr_fail(IVR_ERR_ARG, "null argument");
+    const int W = d->state.winwidth, H = d->state.winheight;
+    if (W < 1 || H < 1 || W > 16384 || H > 16384) return ivr_fail(IVR_ERR_ARG, "frame %dx%d", W, H);
+    int rc = check_desc(v, tf, d, W, H);
+    if (rc) return rc;
+    if (!ctx->scratch) return ivr_fail(IVR_ERR_HIP, "no scratch address for device %d", ctx->device);
+    /* scratch layout: counters | block maxima | run counts | partial planes, 256-byte aligned */
+    const size_t np = (size_t)W * H, nblk_max = (np + IVR_FRAME_BLOCK - 1) / IVR_FRAME_BLOCK + 8 * 1024;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_blk = up(IVR_CTR_BYTES), o_cnt = o_blk + up(nblk_max * sizeof(int)), o_part = o_cnt + up(np * 4);
+    if (o_part > IVR_SCRATCH_BYTES) return ivr_fail(IVR_ERR_ARG, "frame %dx%d exceeds the scratch array", W, H);
+    ivr_frame f; /* a frame over the caller's image and the scratch array */
+    f.W = W;
+    f.H = H;
+    f.d_rgba8 = (uchar4 *)d_rgba8;
+    f.d_rgba32 = (float4 *)d_rgba32;
+    f.d_counters = (unsigned long long *)ctx->scratch;
+    f.d_blockmax = (int *)(ctx->scratch + o_blk);
+    f.blockmax_cap = nblk_max;
+    f.d_runcount = (unsigned *)(ctx->scratch + o_cnt);
+    f.d_tiles = (float4 *)(ctx->scratch + o_part);
+    f.tiles_cap = (IVR_SCRATCH_BYTES - o_part) / sizeof(float4);
+    return render_into(ctx, v, tf, d, &f, true);
 }

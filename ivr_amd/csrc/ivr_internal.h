@@ -53,6 +53,9 @@ struct ivr_ctx {
     int wave_w = 0;
     int block_w = 0; /* pixel block of a workgroup: block_w x (256 / block_w) */
     int sl_split = 1; /* sort-last as one workgroup per (pixel block, run) */
+    /* this device's copy of the library's static scratch array (ivr_render_noalloc),
+     * found by hipGetSymbolAddress when the context is created */
+    unsigned char *scratch = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev[16] = {};
     unsigned long long bytes = 0;
@@ -215,6 +218,8 @@ int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip
 int ivr_launch_blockruns(ivr_ctx *ctx, RenderArgs &a);
 int ivr_launch_combine_runs(ivr_ctx *ctx, const RenderArgs &a, int runs);
 int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles);
+/* n u64 words set to zero by a kernel (no memset node, so callers may capture it) */
+int ivr_launch_zero_u64(hipStream_t stream, unsigned long long *p, size_t n);
 
 /* volume kernels (volume.hip) */
 int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *vol);

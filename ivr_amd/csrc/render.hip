@@ -24,6 +24,8 @@
  * an explicit fmaf, as in the CPU restatement, so results match it bit for
  * bit.
  */
+#include <algorithm>
+
 #include "render_dev.h"
 
 namespace {
@@ -339,6 +341,11 @@ __global__ __launch_bounds__(512) void k_cells(const float2 *range, unsigned cha
     code[c] = v;
 }
 
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long *p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) p[i] = 0ull;
+}
+
 size_t smem_bytes(const RenderArgs &a, int runs = 0)
 {
     return (size_t)a.lutsize * sizeof(float4) + (size_t)((a.nbricks + 1) / 2) * sizeof(float4) +
@@ -490,6 +497,15 @@ int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles)
     hipLaunchKernelGGL(k_composite, g, dim3(256), 0, ctx->stream, fr->d_tiles,
                        (const TileInfo *)fr->d_tilemeta, ntiles, fr->d_rgba8, fr->d_rgba32, fr->W,
                        fr->H);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
+int ivr_launch_zero_u64(hipStream_t stream, unsigned long long *p, size_t n)
+{
+    if (!n) return IVR_OK;
+    const unsigned g = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_zero_u64, dim3(g), dim3(256), 0, stream, p, n);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
 }
