@@ -222,6 +222,11 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
                            const ivr_frame_desc *desc, float threshold_db, float *psnr_out,
                            int *nfallback);
 int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned char *flags);
+/* the network value the renderer composites (encoding from the half-precision
+ * tables, bf16 MFMA MLP: the device function of ivr_surrogate_render) at n
+ * caller points: local xyz in [0,1]^3 of brick bricks[i]; y_out host, n floats */
+int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const float *xyz, const int *bricks,
+                              float *y_out);
 /* versioned binary checkpoint of parameters and Adam state */
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);

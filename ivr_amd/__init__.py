@@ -131,6 +131,7 @@ def _load():
         "ivr_surrogate_render": (C.c_int, [P, P, P, P, C.POINTER(FrameDesc), P]),
         "ivr_surrogate_evaluate": (C.c_int, [P, P, P, P, C.POINTER(FrameDesc), C.c_float, P, P]),
         "ivr_surrogate_set_fallback": (C.c_int, [P, P, P]),
+        "ivr_surrogate_eval_points": (C.c_int, [P, P, C.c_int, P, P, P]),
         "ivr_surrogate_save": (C.c_int, [P, P, C.c_char_p]),
         "ivr_surrogate_load": (C.c_int, [P, P, C.c_char_p]),
         "ivr_batcher_create": (C.c_int, [P, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
@@ -446,6 +447,14 @@ class Surrogate:
         check(lib.ivr_surrogate_evaluate(self.ctx.h, self.h, vol.h, tf.h, C.byref(desc), threshold_db,
                                          _ptr(psnr), C.byref(nf)))
         return psnr, nf.value
+
+    def eval_points(self, xyz, bricks):
+        """the renderer's network value at local points xyz (n, 3) of bricks (n,)"""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        bricks = np.ascontiguousarray(bricks, np.int32)
+        y = np.zeros(len(bricks), np.float32)
+        check(lib.ivr_surrogate_eval_points(self.ctx.h, self.h, len(bricks), _ptr(xyz), _ptr(bricks), _ptr(y)))
+        return y
 
     def set_fallback(self, flags):
         f = np.ascontiguousarray(flags, np.uint8)

@@ -234,6 +234,21 @@ __device__ __forceinline__ float own(const float v[4])
     return g == 0 ? v[0] : (g == 1 ? v[1] : (g == 2 ? v[2] : v[3]));
 }
 
+/* The network value at each lane's sample (local position x in [0,1]^3 of
+ * brick b) as the renderer evaluates it: encoding from the half-precision
+ * tables, rounded to bf16 for the first layer, MLP on MFMA.  Every lane of
+ * the wavefront must call it (lanes without a sample pass any valid point). */
+__device__ __forceinline__ float sg_infer(const Enc &enc, const Mlp &mlp, const float x[3], int b)
+{
+    SampleSet ss;
+    gather_samples(x, b, ss);
+    bf16x8 E[4];
+    encode_frags<__half2>(enc, ss, E);
+    Fwd f;
+    mlp_forward(mlp, E, f);
+    return own(f.y);
+}
+
 /* ---------------------------------------------------------------- training */
 
 struct TrainArgs {
@@ -900,13 +915,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
             x[1] = ((float)p.i + p.fy - (float)(by << a.bshift)) * invB;
             x[2] = ((float)p.k + p.fz - (float)(bz << a.bshift)) * invB;
         }
-        SampleSet ss;
-        gather_samples(x, b, ss);
-        bf16x8 E[4];
-        encode_frags<__half2>(S.enc, ss, E);
-        Fwd f;
-        mlp_forward(S.mlp, E, f);
-        const float yv = own(f.y);
+        const float yv = sg_infer(S.enc, S.mlp, x, b);
         if (live) {
             if (S.only_brick < 0 || S.only_brick == b) {
                 float s = yv;
@@ -959,6 +968,24 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg_items(const SgRen
     const int blk = (int)blockIdx.x - prefix[q];
     if (S.skip) sg_frame_block<true>(S, blk, smem);
     else sg_frame_block<false>(S, blk, smem);
+}
+
+/* the renderer's network value (sg_infer) at caller points: one lane per
+ * point, padding lanes evaluate a dummy point */
+__global__ __launch_bounds__(256) void k_sg_points(const Enc enc, const Mlp mlp, int n, const float *__restrict__ xs,
+                                                   const int *__restrict__ bs, float *__restrict__ ys)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    float x[3] = {0.5f, 0.5f, 0.5f};
+    int b = 0;
+    if (t < n) {
+        x[0] = xs[3 * t];
+        x[1] = xs[3 * t + 1];
+        x[2] = xs[3 * t + 2];
+        b = bs[t];
+    }
+    const float y = sg_infer(enc, mlp, x, b);
+    if (t < n) ys[t] = y;
 }
 
 /* sum of squared RGBA8 differences */
@@ -1123,6 +1150,30 @@ int ivr_surrogate_render(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume 
             if (c) return ivr_fail(IVR_ERR_ARG, "a brick falls back to the plain volume, which was not given");
     }
     return sg_launch(ctx, s, vol, tf, d, fr, s->d_fallback, -1);
+}
+
+int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const float *xyz, const int *bricks,
+                              float *y_out)
+{
+    if (!ctx || !s || n < 1 || !xyz || !bricks || !y_out) return ivr_fail(IVR_ERR_ARG, "bad argument");
+    for (int i = 0; i < n; i++)
+        if (bricks[i] < 0 || bricks[i] >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", bricks[i]);
+    ivr_dscope io(ctx);
+    int rc = io.alloc((size_t)n * 20);
+    if (rc) return rc;
+    float *dx = (float *)io.p, *dy = dx + 3 * (size_t)n;
+    int *db = (int *)(dy + n);
+    IVR_HIP(hipMemcpyAsync(dx, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    IVR_HIP(hipMemcpyAsync(db, bricks, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    Mlp m;
+    m.frag = (const bf16x8 *)s->frag;
+    m.p = s->params;
+    hipLaunchKernelGGL(k_sg_points, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, make_enc(s), m, n,
+                       (const float *)dx, (const int *)db, dy);
+    IVR_HIP(hipGetLastError());
+    IVR_HIP(hipMemcpyAsync(y_out, dy, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
 }
 
 int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned char *flags)

@@ -105,7 +105,140 @@ def test_adam_first_step_closed_form():
     assert np.allclose(m1, 0.1 * g, rtol=1e-6) and np.allclose(v1, 0.01 * g * g, rtol=1e-5)
 
 
+def _corner_coords(T, l, n1, hashed):
+    """per table entry of level l, the vertex coordinates it holds: dense
+    entries decode x + n1 (y + n1 z); a hashed entry's place in its 64-entry
+    slot decodes the Morton interleave x0 y0 z0 x1 y1 z1 to the vertex's
+    coordinates inside its 4x4x4 block"""
+    e = np.arange(T, dtype=np.int64)
+    if not hashed:
+        return e % n1, (e // n1) % n1, e // (n1 * n1)
+    loc = e & 63
+    bit = lambda k: (loc >> k) & 1  # noqa: E731
+    return bit(0) | bit(3) << 1, bit(1) | bit(4) << 1, bit(2) | bit(5) << 1
+
+
+def test_encoding_reproduces_affine_fields():
+    """a level whose entries hold an affine function of their vertex's
+    coordinates encodes every point as that function of x * N (dense levels:
+    global coordinates; hashed levels: coordinates inside the 4x4x4 block,
+    for cells inside one block, whatever slots the blocks share); a wrong
+    corner weight, axis or Morton bit breaks it"""
+    T = 1 << 17  # levels 0-2 dense (17^3, 25^3, 37^3 entries), 3-15 hashed
+    rng = np.random.default_rng(5)
+    table = np.zeros((16, T, 2), np.float32)
+    coef = rng.integers(-64, 65, (16, 2, 4)) / 64.0  # a.(x, y, z) + c per feature, exact in float32 entries
+    hashed = []
+    for l in range(16):
+        N = oracle.lib.ivros_level_res(l, 16, 1.5)
+        h = (N + 1) ** 3 > T
+        hashed.append(h)
+        x, y, z = _corner_coords(T, l, N + 1, h)
+        for k in range(2):
+            a = coef[l, k]
+            table[l, :, k] = (a[0] * x + a[1] * y + a[2] * z + a[3]).astype(np.float32)
+    assert hashed[:3] == [False] * 3 and all(hashed[3:])
+    checked = [0] * 16
+    for x in rng.uniform(0.0, 1.0, (300, 3)):
+        enc = oracle.sg_encode(table, T, x)
+        for l in range(16):
+            N = oracle.lib.ivros_level_res(l, 16, 1.5)
+            u = x * N
+            v0 = np.minimum(np.floor(u).astype(int), N - 1)
+            if hashed[l]:
+                if ((v0 & 3) == 3).any():
+                    continue  # the cell spans two blocks
+                u = u - 4 * (v0 // 4)
+            for k in range(2):
+                a = coef[l, k]
+                assert abs(enc[2 * l + k] - (a[:3] @ u + a[3])) < 1e-9, (l, k, x)
+            checked[l] += 1
+    assert min(checked) > 50
+
+
+def _mlp64(p, e):
+    """W3 relu(W2 relu(W1 e + b1) + b2) + b3 in float64, parameters in the
+    W1 b1 W2 b2 W3 b3 order"""
+    p = p.astype(np.float64)
+    W1, b1 = p[:2048].reshape(64, 32), p[2048:2112]
+    W2, b2 = p[2112:6208].reshape(64, 64), p[6208:6272]
+    W3, b3 = p[6272:6336], p[6336]
+    h1 = np.maximum(W1 @ e + b1, 0.0)
+    h2 = np.maximum(W2 @ h1 + b2, 0.0)
+    return W3 @ h2 + b3
+
+
+def test_mlp_forward_is_the_textbook_mlp():
+    """the oracle's double-precision forward (encoding then MLP) equals
+    numpy's W3 relu(W2 relu(W1 x + b1) + b2) + b3 on its encodings; with the
+    bf16 rounding points on it stays within bf16 precision of it"""
+    rng = np.random.default_rng(8)
+    table = rng.normal(0.0, 0.2, (16, T_SMALL, 2)).astype(np.float32)
+    p = _params(rng)
+    xyz = rng.uniform(0.0, 1.0, (64, 3))
+    ref = np.array([_mlp64(p, oracle.sg_encode(table, T_SMALL, x)) for x in xyz])
+    y64 = oracle.sg_eval(p, table, T_SMALL, xyz, bf16=False)
+    assert np.abs(y64 - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+    y16 = oracle.sg_eval(p, table, T_SMALL, xyz, bf16=True)
+    assert np.abs(y16 - ref).max() < 3e-2 * max(1.0, np.abs(ref).max())
+    assert np.abs(y16 - ref).max() > 0.0  # the rounding points are really on
+
+
+def test_psnr_ssim_closed_forms():
+    """PSNR of an image against itself offset by d is 10 log10(255^2 / d^2);
+    SSIM of an image with itself is 1, and of two constant images a and b it
+    is (2ab + C1) / (a^2 + b^2 + C1)"""
+    rng = np.random.default_rng(9)
+    img = rng.integers(10, 200, (40, 48, 4)).astype(np.uint8)
+    for d in (1, 3, 17):
+        assert abs(ivr.psnr(img + np.uint8(d), img) - 10.0 * np.log10(255.0 ** 2 / d ** 2)) < 1e-9
+    assert ivr.psnr(img, img) == 99.0
+    assert abs(ivr.ssim(img, img) - 1.0) < 1e-12
+    a, b = 80.0, 120.0
+    ia = np.full((30, 30, 4), a, np.uint8)
+    ib = np.full((30, 30, 4), b, np.uint8)
+    c1 = (0.01 * 255) ** 2
+    # luminance weights sum to 1, so a constant RGB image has luminance a
+    assert abs(ivr.ssim(ia, ib) - (2 * a * b + c1) / (a * a + b * b + c1)) < 1e-9
+
+
 # ---------------------------------------------------------------- GPU
+
+
+@pytest.mark.gpu
+def test_render_network_value_matches_oracle(ctx, small_vol):
+    """the network value the renderer composites (ivr_surrogate_eval_points:
+    the device function of k_render_sg, half-precision tables, bf16 MFMA)
+    equals the oracle's forward pass on the fp16-rounded tables, within the
+    bf16 tolerance of the training-path check, at points of several bricks"""
+    vol, _ = small_vol
+    sg = ctx.surrogate(vol, log2_table=12)
+    rng = np.random.default_rng(31)
+    p = _params(rng)
+    tables = [rng.uniform(-0.5, 0.5, (16, T_SMALL, 2)).astype(np.float32) for _ in range(3)]
+    sg.set(mlp=p, table=tables[0], brick=0)
+    sg.set(table=tables[1], brick=3)
+    sg.set(table=tables[2], brick=7)
+    n = 300
+    xyz = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    xyz[:8] = [[0, 0, 0], [1, 1, 1], [0.5, 0.5, 0.5], [1, 0, 0.25], [0, 1, 0.75], [0.999, 0.001, 0.5],
+               [0.25, 0.75, 1.0], [1 / 3, 2 / 3, 0.1]]
+    bricks = np.array([0, 3, 7], np.int32)[rng.integers(0, 3, n)]
+    y = sg.eval_points(xyz, bricks)
+    yo = np.zeros(n)
+    for b, t in zip((0, 3, 7), tables):
+        sel = bricks == b
+        yo[sel] = oracle.sg_eval(p, t.astype(np.float16).astype(np.float32), T_SMALL, xyz[sel].astype(np.float64))
+    err = np.abs(y - yo).max()
+    print("max |y_render - y_oracle| = %.3e (|y| up to %.3f)" % (err, np.abs(yo).max()))
+    assert err < 2e-3 * max(1.0, np.abs(yo).max())
+    # the fp32 tables would differ from the half copy: the check sees which one the renderer reads
+    yf = np.zeros(n)
+    for b, t in zip((0, 3, 7), tables):
+        sel = bricks == b
+        yf[sel] = oracle.sg_eval(p, t, T_SMALL, xyz[sel].astype(np.float64))
+    assert np.abs(y - yo).mean() < np.abs(y - yf).mean()
+    sg.close()
 
 @pytest.fixture(scope="module")
 def small_vol(ctx):
