@@ -25,10 +25,14 @@ reference_cpu: the reference ray.c renderer compiled from its sources
 core, against this library's compat mode on the GPU over the same volume,
 camera and LUT (bit-identical images).
 
---config 4 (learned surrogate of the C2 volume), --config 5 (steering
-service) and --gradient (the central-difference gradient field of the C2
-volume) print their own JSON lines; they are measurements of those parts,
-not the headline.
+The default run also carries the rest of the metric in the same line:
+"survey_tf" (the headline frame with control-point RGBA U[0,1]),
+"surrogate" (frames/s at 1920x1080 with per-brick fallback below 40 dB, for
+the surrogate of the C2 volume and of the 1024^3 fp16 volume, each trained
+on the GPU for --train-seconds) and "serving" (p50/p99 latency and frames/s
+of the steering mix at 50-400 req/s, plain and with the 1024^3 resident
+served from its surrogate).  --config 4, --config 5 and --gradient (the
+central-difference gradient field of the C2 volume) print their own lines.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
 """
@@ -60,7 +64,11 @@ def main():
                     help="brick compositing (default per config)")
     ap.add_argument("--no-cpu", action="store_true", help="leave out the CPU legs")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--train-seconds", type=float, default=60.0, help="--config 4/5: training time budget")
+    ap.add_argument("--train-seconds", type=float, default=None,
+                    help="surrogate training time budget per surrogate (default: 20 s in the headline run, "
+                         "60 s with --config 4/5)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="the C2 frame rate alone, without the survey-TF, surrogate and serving parts")
     ap.add_argument("--sg-volume", type=int, default=2, choices=[2, 3],
                     help="--config 4: train on the C2 (512^3 fp32) or the C3 (1024^3 fp16) volume")
     ap.add_argument("--serve-surrogate", action="store_true",
@@ -86,6 +94,8 @@ def main():
     import ivr_amd as ivr
     from ivr_amd import configs
 
+    if args.train_seconds is None:
+        args.train_seconds = 60.0 if args.config in (4, 5) else 20.0
     if args.gradient:
         bench_gradient(args)
         return
@@ -221,11 +231,73 @@ def main():
     }
     if cpu and cpu["value"] > 0:
         out["config"]["gpu_over_cpu"] = round(gsps / cpu["value"], 1)
+    if c == 2 and not args.headline_only:
+        # the rest of the metric: the same frame under the survey's control-point opacities,
+        # surrogate frames/s at PSNR >= 40 dB (512^3 and 1024^3), steering p50/p99 and frames/s
+        out["survey_tf"] = survey_tf_measure(ctx, vol, st, comp, W, H, cfg["brick"], args.steps)
+        fr.close()
+        tf.close()
+        sur = {}
+        sur["c4_512"], _ = surrogate_measure(ctx, vol, 2, args.train_seconds, args.steps,
+                                             0.0 if args.no_cpu else 5.0)
+        cfg3 = configs.WORKLOADS[3]
+        v3 = ctx.volume(*cfg3["dims"], cfg3["dtype"], brick=cfg3["brick"])
+        v3.generate(configs.gen_params(3))
+        sur["vol_1024"], sg3 = surrogate_measure(ctx, v3, 3, args.train_seconds, args.steps, keep=True)
+        out["surrogate"] = sur
+        vols = serving_volumes(ctx, c2=vol, c3=v3)
+        n_req = 200
+        plain_rows, _ = serving_sweep(ctx, vols, n_req, (50.0, 100.0, 200.0, 400.0, 1e9))
+        sg_rows, _ = serving_sweep(ctx, vols, n_req, (50.0, 100.0, 1e9), sg3, "surrogate")
+        out["serving"] = {
+            "workload": "C5: 256^3 fp32, 512^3 fp32, 1024^3 fp16 resident; 16 slots; 64 GiB budget; %d seeded "
+                        "requests per rate, resolutions {256^2, 512^2, 1024^2, 1920x1080}" % n_req,
+            "plain": plain_rows, "capacity_frames_per_s": plain_rows[-1]["frames_per_s"],
+            "p99_ms_at_400": plain_rows[3]["p99_ms"],
+            "with_1024_surrogate": sg_rows, "surrogate_capacity_frames_per_s": sg_rows[-1]["frames_per_s"]}
+        sg3.close()
+        vols[0].close()
+        v3.close()
+    else:
+        fr.close()
+        tf.close()
     print(json.dumps(out), flush=True)
-    fr.close()
-    tf.close()
     vol.close()
     ctx.close()
+
+
+def survey_tf_measure(ctx, vol, st, comp, W, H, brick, steps):
+    """The headline frame with control-point RGBA drawn U[0,1] (the value-0
+    point transparent, so the zeroed part stays empty space) instead of
+    opacities capped at 0.05: ms/frame, Gsamples/s, and the GPU image against
+    the oracle over a band of rows through the centre"""
+    import numpy as np
+    import ivr_amd as ivr
+    import oracle
+    lut = ivr.tf_build(2002, 1.0)
+    tf = ctx.tf(lut)
+    fr = ctx.frame(W, H)
+    d = ivr.config_desc(st, composite=comp)
+    ctx.render(vol, tf, d, fr)
+    ctx.sync()
+    ctx.record(0)
+    for _ in range(steps):
+        ctx.render(vol, tf, d, fr)
+    ctx.record(1)
+    ctx.sync()
+    ms = ctx.elapsed(0, 1) / steps
+    S = fr.stats()["visited"]
+    img = fr.download()
+    r0, r1 = H // 2 - 16, H // 2 + 16
+    ref32, s_band = _oracle_frame(oracle, vol.download(), st, lut, comp, brick, (r0, r1), cpu_threads())
+    dmax = int(np.abs(oracle.quantize(ref32[r0:r1]).astype(int) - img[r0:r1].astype(int)).max())
+    fr.close()
+    tf.close()
+    log("survey TF: %.4f ms/frame, %d samples, %.1f Gsamples/s; max |gpu - oracle| %d/255 on rows %d-%d"
+        % (ms, S, S / ms / 1e6, dmax, r0, r1))
+    return {"tf": "5 control points, RGBA U[0,1] (value 0 transparent)", "ms_per_frame": round(ms, 4),
+            "samples_per_frame": int(S), "gsamples_s": round(S / ms / 1e6, 2),
+            "max_abs_diff_vs_oracle": dmax, "oracle_rows": [r0, r1]}
 
 
 def kernel_ms(ctx, render, fr, steps):
@@ -267,13 +339,80 @@ def train_surrogate(ctx, vol, seconds, batch=1 << 24, chunk=5):
     return sg, losses, steps, t_train
 
 
-def bench_surrogate(args):
-    """C4: per-brick hash grids + BF16 MFMA MLP trained on the C2 volume
-    (or, with --sg-volume 3, the 1024^3 fp16 C3 volume) with 2^24 samples
-    per Adam step for a time budget, then 1920x1080 frames from the
-    surrogate against the plain render: PSNR/SSIM, frames/s, and the
-    per-brick fallback below 40 dB."""
+def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=False):
+    """A surrogate of `vol` (config c's volume: 2 = the 512^3 fp32 C2 volume,
+    3 = the 1024^3 fp16 C3 volume): per-brick hash grids (16 levels x 2^19 x
+    2 features) + the BF16 MFMA MLP trained on the GPU with 2^24 samples per
+    Adam step for a time budget, then 1920x1080 frames from it against the
+    plain render of the same camera: PSNR/SSIM, frames/s, per-brick fallback
+    below 40 dB.  Returns (dict, surrogate or None)."""
     import numpy as np
+    import ivr_amd as ivr
+    from ivr_amd import configs
+    W, H = 1920, 1080
+    tf = ctx.tf(configs.tf_lut(c))
+    desc = ivr.config_desc(ivr.orbit_state(3004, W, H), skip=True)  # camera seed of C4; skipping on
+    fr = ctx.frame(W, H)
+    plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), steps)
+    plain = fr.download()
+    batch = 1 << 24
+    sg, losses, nsteps, t_train = train_surrogate(ctx, vol, train_seconds, batch)
+    step_ms = t_train * 1e3 / nsteps
+    sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
+    sg_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), steps)
+    sg_stats = fr.stats()
+    img = fr.download()
+    p_sg, s_sg = ivr.psnr(img, plain), ivr.ssim(img, plain)
+    bpsnr, nfall = sg.evaluate(vol, tf, desc, threshold_db=40.0)
+    fb_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), steps)
+    img_fb = fr.download()
+    p_fb, s_fb = ivr.psnr(img_fb, plain), ivr.ssim(img_fb, plain)
+    # the network kernel's own duration (HIP events around k_render_sg on the library's stream)
+    k_ms, _ = kernel_ms(ctx, lambda: sg.render(vol, tf, desc, fr), fr, steps)
+    S = int(fr.stats()["visited"])
+    log("surrogate %s: train %.1f ms/step (%d steps), render %.3f ms (plain %.3f); PSNR %.2f dB SSIM %.4f; "
+        "%d/%d bricks below 40 dB -> with fallback PSNR %.2f dB, %.3f ms"
+        % ("C2" if c == 2 else "1024^3", step_ms, nsteps, sg_ms, plain_ms, p_sg, s_sg, nfall, sg.nbricks, p_fb,
+           fb_ms))
+    flops = 2 * (32 * 64 + 64 * 64 + 64)
+    g_bytes = S * 16 * 8 * 4  # per network sample: 16 levels x 8 corners x 2 half-precision features
+    workload = "surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1, 1920x1080" \
+        % ("C2 (512^3 fp32)" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks)
+    out = {
+        "workload": workload,
+        "fps": round(1000.0 / fb_ms, 2), "ms_per_frame": round(fb_ms, 3),
+        "psnr_db": round(p_fb, 2), "ssim": round(s_fb, 4),
+        "fallback_bricks": int(nfall), "bricks": sg.nbricks,
+        "brick_psnr_db": [round(float(x), 2) for x in bpsnr],
+        "no_fallback": {"ms_per_frame": round(sg_ms, 3), "psnr_db": round(p_sg, 2), "ssim": round(s_sg, 4)},
+        "network_samples_per_frame": int(sg_stats["visited"]),
+        "plain_ms_per_frame": round(plain_ms, 3), "plain_fps": round(1000.0 / plain_ms, 2),
+        "train": {"seconds": round(t_train, 1), "steps": nsteps, "samples_per_step": batch,
+                  "ms_per_step": round(step_ms, 2),
+                  "mlp_tflops": round(batch * 3 * flops / (step_ms * 1e-3) / 1e12, 2),
+                  "loss_first": losses[0], "loss_last": losses[-1]},
+        "roofline": {"bound": "hbm", "kernel": "k_render_sg<true>", "kernel_ms": round(k_ms, 4),
+                     "achieved": round(g_bytes / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": round(g_bytes / (k_ms * 1e-3) / HBM_PEAK, 4),
+                     "traffic": measured_traffic("C4 surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, "
+                                                 "MLP 32-64-64-1" % ("C2" if c == 2 else "C3", sg.nbricks),
+                                                 "surrogate", True),
+                     "mfma_tflops": round(S * flops / (k_ms * 1e-3) / 1e12, 2), "mfma_peak_tflops": 2500.0,
+                     "mfma_frac": round(S * flops / (k_ms * 1e-3) / 2.5e15, 5)},
+    }
+    if cpu_seconds > 0:
+        out["cpu_baseline"] = cpu_surrogate_baseline(sg, cpu_seconds)
+    fr.close()
+    tf.close()
+    if not keep:
+        sg.close()
+        sg = None
+    return out, sg
+
+
+def bench_surrogate(args):
+    """C4 alone (--config 4): the surrogate of the C2 volume, or with
+    --sg-volume 3 of the 1024^3 fp16 C3 volume"""
     import ivr_amd as ivr
     from ivr_amd import configs
     c = args.sg_volume
@@ -281,80 +420,20 @@ def bench_surrogate(args):
     ctx = ivr.Context(0)
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(c))
-    W, H = 1920, 1080
-    lut = configs.tf_lut(c)
-    tf = ctx.tf(lut)
-    desc = ivr.config_desc(ivr.orbit_state(3004, W, H))  # camera seed of C4
-    fr = ctx.frame(W, H)
-    plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), args.steps)
-    plain = fr.download()
-    batch = 1 << 24
-    sg, losses, steps, t_train = train_surrogate(ctx, vol, args.train_seconds, batch)
-    step_ms = t_train * 1e3 / steps
-    sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
-    ns_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
-    p_ns = ivr.psnr(fr.download(), plain)
-    desc = ivr.config_desc(desc.state, skip=True)  # the network sees only samples of non-transparent cells
-    sg_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
-    sg_stats = fr.stats()
-    img = fr.download()
-    p_sg, s_sg = ivr.psnr(img, plain), ivr.ssim(img, plain)
-    bpsnr, nfall = sg.evaluate(vol, tf, desc, threshold_db=40.0)
-    fb_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), args.steps)
-    img_fb = fr.download()
-    p_fb, s_fb = ivr.psnr(img_fb, plain), ivr.ssim(img_fb, plain)
-    log("surrogate: train %.1f ms/step (%d steps), render %.3f ms (plain %.3f); PSNR %.2f dB SSIM %.4f; "
-        "%d/%d bricks below 40 dB -> with fallback PSNR %.2f dB, %.3f ms"
-        % (step_ms, steps, sg_ms, plain_ms, p_sg, s_sg, nfall, sg.nbricks, p_fb, fb_ms))
-    flops_step = batch * 3 * 2 * (32 * 64 + 64 * 64 + 64)
-    workload = "C4 surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1" \
-        % ("C2" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks)
-    # roofline of the network kernel: per network sample 16 levels x 8 corners x 2 half-precision
-    # features gathered (512 B) and 2 x (32x64 + 64x64 + 64) = 12,416 flops, over the device time of
-    # the frame's macrocell pass + k_render_sg (HIP events)
-    k_ms, _ = kernel_ms(ctx, lambda: sg.render(vol, tf, desc, fr), fr, args.steps)
-    S = int(fr.stats()["visited"])
-    g_bytes = S * 16 * 8 * 4
-    flops = S * 2 * (32 * 64 + 64 * 64 + 64)
-    roofline = {"bound": "hbm", "kernel": "k_render_sg<true>", "kernel_ms": round(k_ms, 4),
-                "achieved": round(g_bytes / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(g_bytes / (k_ms * 1e-3) / HBM_PEAK, 4),
-                "traffic": measured_traffic(workload, "surrogate", True),
-                "mfma_tflops": round(flops / (k_ms * 1e-3) / 1e12, 2), "mfma_peak_tflops": 2500.0,
-                "mfma_frac": round(flops / (k_ms * 1e-3) / 2.5e15, 5)}
-    cpu = None if args.no_cpu else cpu_surrogate_baseline(sg, args.cpu_seconds)
+    res, _ = surrogate_measure(ctx, vol, c, args.train_seconds, args.steps, 0.0 if args.no_cpu else args.cpu_seconds)
     out = {
         "metric": "surrogate frames/s at 1920x1080 with per-brick fallback below 40 dB",
-        "value": round(1000.0 / fb_ms, 2),
+        "value": res["fps"],
         "unit": "frames/s",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "bf16 MLP on MFMA, fp32 tables and accumulation",
-        "data": "synthetic %s volume (%s), surrogate trained on the GPU"
-                % ("C2" if c == 2 else "C3", "512^3 fp32" if c == 2 else "1024^3 fp16"),
-        "config": {
-            "workload": workload,
-            "image": "%dx%d" % (W, H),
-            "train_steps": steps, "train_samples_per_step": batch, "train_ms_per_step": round(step_ms, 2),
-            "train_mlp_tflops": round(flops_step / (step_ms * 1e-3) / 1e12, 2),
-            "loss_first": losses[0], "loss_last": losses[-1],
-            "surrogate_ms": round(sg_ms, 3), "surrogate_fps": round(1000.0 / sg_ms, 2),
-            "surrogate_no_skip_ms": round(ns_ms, 3), "surrogate_no_skip_psnr_db": round(p_ns, 2),
-            "network_samples_per_frame": int(sg_stats["visited"]),
-            "network_gsamples_s": round(sg_stats["visited"] / sg_ms / 1e6, 3),
-            "psnr_db": round(p_sg, 2), "ssim": round(s_sg, 4),
-            "brick_psnr_db": [round(float(x), 2) for x in bpsnr],
-            "fallback_bricks": int(nfall),
-            "fallback_ms": round(fb_ms, 3), "fallback_psnr_db": round(p_fb, 2), "fallback_ssim": round(s_fb, 4),
-            "plain_ms": round(plain_ms, 3), "plain_fps": round(1000.0 / plain_ms, 2),
-        },
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "data": "synthetic %s volume, surrogate trained on the GPU" % ("C2" if c == 2 else "C3"),
+        "config": res,
+        "roofline": res.pop("roofline"),
+        "cpu_baseline": res.pop("cpu_baseline", None),
     }
     print(json.dumps(out), flush=True)
-    sg.close()
-    fr.close()
-    tf.close()
     vol.close()
     ctx.close()
 
@@ -385,63 +464,68 @@ def cpu_surrogate_baseline(sg, target_s):
                       "rounding), one timed run" % n}
 
 
-def bench_serving(args):
-    """C5: three resident volumes (256^3 fp32, C2 512^3 fp32, C3 1024^3 fp16)
-    behind the continuous batcher; Poisson arrivals at 50..400 req/s and an
-    all-at-once burst for capacity; each request a seeded orbit camera, a
-    jittered TF and a resolution from {256^2, 512^2, 1024^2, 1920x1080}."""
+def serving_volumes(ctx, c2=None, c3=None):
+    """the three resident volumes of the steering mix: 256^3 fp32, the C2
+    512^3 fp32 and the C3 1024^3 fp16 volume (generated unless given)"""
     import ivr_amd as ivr
-    from ivr_amd import configs, serving
-    import numpy as np
-    ctx = ivr.Context(0)
+    from ivr_amd import configs
     vols = []
-    for (n, dt, c, brick) in ((256, ivr.IVR_F32, 2, 256), (512, ivr.IVR_F32, 2, 256), (1024, ivr.IVR_F16, 3, 256)):
-        v = ctx.volume(n, n, n, dt, brick=brick)
+    for (n, dt, c, given) in ((256, ivr.IVR_F32, 2, None), (512, ivr.IVR_F32, 2, c2), (1024, ivr.IVR_F16, 3, c3)):
+        if given is not None:
+            vols.append(given)
+            continue
+        v = ctx.volume(n, n, n, dt, brick=256)
         v.generate(configs.gen_params(c))
         vols.append(v)
+    return vols
 
-    def sweep(bt, tag):
-        rows = []
-        for k, rate in enumerate((50.0, 100.0, 200.0, 400.0, 1e9)):
-            reqs = serving.make_requests(args.requests, len(vols), seed=4100 + k, tf_seed=2005)
-            res = serving.run_poisson(bt, reqs, rate, seed=4000 + k)
-            row = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in res.items() if kk != "records"}
-            row["rate"] = "burst" if rate >= 1e9 else rate
-            # latency of the requests of the 1024^3 resident (volume 2) apart
-            lat3 = [(r["t_done"] - r["t_arrival"]) * 1e3 for r in res["records"].values()
-                    if r["status"] == 0 and reqs[r["id"] - 1]["volume"] == 2]
-            if lat3:
-                row["vol1024_p50_ms"] = round(float(np.percentile(lat3, 50)), 3)
-                row["vol1024_p99_ms"] = round(float(np.percentile(lat3, 99)), 3)
-            log("serving (%s): %s" % (tag, row))
-            rows.append(row)
-        return rows
 
+def serving_sweep(ctx, vols, n_requests, rates, sg=None, tag="plain"):
+    """Poisson arrivals at each rate (req/s; >= 1e9: all at once, the
+    capacity) into the continuous batcher with the three residents (the
+    1024^3 one served from `sg` when given); each request a seeded orbit
+    camera, a jittered TF, a resolution from {256^2, 512^2, 1024^2,
+    1920x1080} and a volume; latency = arrival -> RGBA8 on the host"""
+    import numpy as np
+    from ivr_amd import serving
     bt = ctx.batcher(64 << 30, 16, 1920, 1080)
-    for v in vols:
-        bt.add_volume(v)
-    rows = sweep(bt, "plain")
+    bt.add_volume(vols[0])
+    bt.add_volume(vols[1])
+    if sg is None:
+        bt.add_volume(vols[2])
+    else:
+        bt.add_surrogate(sg, vols[2])
+    rows = []
+    for k, rate in enumerate(rates):
+        reqs = serving.make_requests(n_requests, len(vols), seed=4100 + k, tf_seed=2005)
+        res = serving.run_poisson(bt, reqs, rate, seed=4000 + k)
+        row = {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in res.items() if kk != "records"}
+        row["rate"] = "burst" if rate >= 1e9 else rate
+        lat3 = [(r["t_done"] - r["t_arrival"]) * 1e3 for r in res["records"].values()
+                if r["status"] == 0 and reqs[r["id"] - 1]["volume"] == 2]
+        if lat3:
+            row["vol1024_p50_ms"] = round(float(np.percentile(lat3, 50)), 3)
+            row["vol1024_p99_ms"] = round(float(np.percentile(lat3, 99)), 3)
+        log("serving (%s): %s" % (tag, row))
+        rows.append(row)
     st = bt.stats()
     bt.close()
+    return rows, st
+
+
+def bench_serving(args):
+    """C5 alone (--config 5): the steering mix at 50..400 req/s and an
+    all-at-once burst, optionally with the 1024^3 resident served from a
+    surrogate trained for --train-seconds (--serve-surrogate)"""
+    import ivr_amd as ivr
+    ctx = ivr.Context(0)
+    vols = serving_volumes(ctx)
+    rows, st = serving_sweep(ctx, vols, args.requests, (50.0, 100.0, 200.0, 400.0, 1e9))
     sg_part = None
     if args.serve_surrogate:
-        # the 1024^3 resident served from its surrogate (fallback bricks from the plain volume)
-        sg, losses, steps, t_train = train_surrogate(ctx, vols[2], args.train_seconds)
-        lut = configs.tf_lut(3)
-        tf = ctx.tf(lut)
-        desc = ivr.config_desc(ivr.orbit_state(3004, 1920, 1080), skip=True)
-        bpsnr, nfall = sg.evaluate(vols[2], tf, desc, threshold_db=40.0)
-        tf.close()
-        bt = ctx.batcher(64 << 30, 16, 1920, 1080)
-        bt.add_volume(vols[0])
-        bt.add_volume(vols[1])
-        bt.add_surrogate(sg, vols[2])
-        srows = sweep(bt, "surrogate")
-        bt.close()
-        sg_part = {"train_steps": steps, "train_s": round(t_train, 1), "loss_last": losses[-1],
-                   "fallback_bricks": int(nfall), "bricks": sg.nbricks,
-                   "brick_psnr_db_min": round(float(bpsnr.min()), 2), "rates": srows,
-                   "capacity_frames_per_s": srows[-1]["frames_per_s"]}
+        res, sg = surrogate_measure(ctx, vols[2], 3, args.train_seconds, args.steps, keep=True)
+        srows, _ = serving_sweep(ctx, vols, args.requests, (50.0, 100.0, 200.0, 400.0, 1e9), sg, "surrogate")
+        sg_part = {"surrogate": res, "rates": srows, "capacity_frames_per_s": srows[-1]["frames_per_s"]}
         sg.close()
     out = {
         "metric": "steering service p99 latency (arrival -> RGBA8 on host) and frames/s",
@@ -452,8 +536,7 @@ def bench_serving(args):
         "data": "synthetic volumes generated on the GPU; seeded requests",
         "config": {"workload": "C5: 256^3 fp32, 512^3 fp32, 1024^3 fp16 resident; 16 slots; 64 GiB budget",
                    "requests_per_rate": args.requests, "rates": rows,
-                   "capacity_frames_per_s": rows[-1]["frames_per_s"],
-                   "batcher": {k: v for k, v in st.items()} if isinstance(st, dict) else None,
+                   "capacity_frames_per_s": rows[-1]["frames_per_s"], "batcher": st,
                    "surrogate_1024": sg_part},
     }
     print(json.dumps(out), flush=True)
