@@ -15,9 +15,11 @@
  * bit-exact with the reference); the frame loop of createlocalimage
  * (main.c:937-1026) becomes one launch through
  * ivr_compat_createlocalimage(rank), which a host replaces its pixel loop
- * with.  The LUT builders of colormap.c are host initialisation code, as in
- * the reference.  The host's volume data is mirrored to the device on first
- * use; call ivr_compat_invalidate() after changing volume.data or dims.
+ * with.  The LUT builders of colormap.c run on the device over the caller's
+ * table (ivr_compat_colormap).  The host's volume data is mirrored to the
+ * device on first use (its value range, which sizes the LUT, is reduced
+ * there too); call ivr_compat_invalidate() after changing volume.data or
+ * dims.
  */
 #ifndef IVR_COMPAT_H
 #define IVR_COMPAT_H
@@ -108,6 +110,11 @@ void nomap(ivr_color4f *lut, int *hist, int hsize, float hmin, float hmax);
 int ivr_compat_createlocalimage(int rank);
 /* compositetiles (main.c:1203-1218) on the GPU */
 int ivr_compat_compositetiles(float *tilebuf, float *rcolorbuf, int num, int w, int h);
+/* colormap builder `id` (colormap.h:9-14; -1 nomap) applied on the device to
+ * the caller's host table of hsize entries, as the builders of colormap.c:21-181
+ * write it (fields they leave alone keep the caller's values); the exported
+ * builders above call it */
+int ivr_compat_colormap(int id, ivr_color4f *lut, int hsize, float hmin, float hmax);
 /* forget the device mirror of the host volume */
 void ivr_compat_invalidate(void);
 

@@ -18,6 +18,10 @@
  *   ivr_volume_upload      readvoldata/readfromfile + brick   main.c:629-706, 1220-1269,
  *                          decomposition (getlocalbrick)     utils.c:332-388
  *   ivr_tf_create          the colour LUT (lut, lutoffset)   main.c:1075-1200
+ *   ivr_volume_histogram   histogram                         mathutil.c:444-470
+ *   ivr_tf_colormap        the colormap builders             colormap.c:21-205
+ *   ivr_volume_luts        initvolluts (histogram, colormap,  main.c:1075-1200
+ *                          lutoffset, maxlutindx)
  *   ivr_state_orbit        the client's STATE producer       ivrclient.c:536-569
  *   ivr_volume_gradient    gradient3d (GRAD)                 mathutil.c:242-338,
  *                                                            main.c:1181-1196
@@ -147,6 +151,28 @@ int ivr_volume_info(const ivr_volume *vol, int dims[3], int nbricks[3], int *bri
 int ivr_tf_create(ivr_ctx *ctx, const float *rgba, int n, ivr_tf **out);
 int ivr_tf_update(ivr_ctx *ctx, ivr_tf *tf, const float *rgba, int n);
 int ivr_tf_destroy(ivr_ctx *ctx, ivr_tf *tf);
+
+/* ---- colour tables built on the device from the volume ---- */
+/* histogram (reference mathutil.c:444-470) of the volume's values: vmin and
+ * vmax (NaN ignored, as fmin/fmax do), hsize = (int)((vmax - vmin) + 1) and,
+ * when hist != NULL and hsize <= cap, hist[i] = the number of voxels v with
+ * (int)(v - vmin) == i */
+int ivr_volume_histogram(ivr_ctx *ctx, const ivr_volume *vol, float *vmin, float *vmax, int *hsize, int *hist,
+                         int cap);
+/* colormap builder `id` (colormap.c:21-181, colormap.h:9-14: BASE 0,
+ * RAINBOW 1, MEDICAL 2, SEISMIC 3, JET 4; NOMAP -1 leaves the entries as
+ * they are) of hsize <= 4096 entries over the value range [hmin, hmax] into
+ * tf, computed on the device with the reference's arithmetic; tf then holds
+ * hsize entries, those the builder leaves unwritten zero */
+int ivr_tf_colormap(ivr_ctx *ctx, ivr_tf *tf, int id, int hsize, float hmin, float hmax);
+/* the LUT's first n entries as RGBA floats into host memory */
+int ivr_tf_download(ivr_ctx *ctx, const ivr_tf *tf, float *rgba, int n);
+/* initvolluts (main.c:1075-1200) as the render server runs it: the
+ * histogram of the volume, colormap `id` over its [min, max] into tf (hsize
+ * entries), *lutoffset = (int)(min + 0.5), *maxlutindx = hsize - 1 (main.c:1158);
+ * hsize_out may be NULL.  IVR_ERR_UNSUPPORTED when hsize exceeds 4096. */
+int ivr_volume_luts(ivr_ctx *ctx, const ivr_volume *vol, int colormap_id, ivr_tf *tf, int *lutoffset,
+                    int *maxlutindx, int *hsize_out);
 
 /* ---- frames: device RGBA8 image (+ optional RGBA32F), row 0 = bottom ---- */
 int ivr_frame_create(ivr_ctx *ctx, int width, int height, int keep_rgba32, ivr_frame **out);
@@ -303,6 +329,14 @@ int ivr_generate_host(const ivr_gen_params *gp, int nx, int ny, int nz, float *o
  * floats native little-endian (connection.c:167-200) */
 void ivr_state_pack(const ivr_state *st, unsigned char out[184]);
 void ivr_state_unpack(const unsigned char in[184], ivr_state *st);
+/* byte-swap the STATE's floats, as recvstate does in a build with BIGENDIAN
+ * (connection.c:191-197) for a peer of the other byte order */
+void ivr_state_swap_floats(ivr_state *st);
+/* writetotiff (utils.c:18-154): a 512-byte big-endian TIFF header and
+ * w x h x planes 8-bit samples (planes 1: red; 3: RGB) of an RGBA32F image
+ * in its row order, each value * 255 truncated, into out (cap bytes);
+ * returns the bytes written (512 + w*h*planes) or -1 */
+long long ivr_tiff_encode(const float *rgba32, int w, int h, int planes, unsigned char *out, size_t cap);
 /* deterministic PRNG used by every generator: SplitMix64-seeded PCG32 */
 uint32_t ivr_rng_u32(uint64_t *state);
 

@@ -106,6 +106,12 @@ def _load():
         "ivr_tf_create": (C.c_int, [P, P, C.c_int, C.POINTER(P)]),
         "ivr_tf_update": (C.c_int, [P, P, P, C.c_int]),
         "ivr_tf_destroy": (C.c_int, [P, P]),
+        "ivr_tf_colormap": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, C.c_float]),
+        "ivr_tf_download": (C.c_int, [P, P, P, C.c_int]),
+        "ivr_volume_histogram": (C.c_int, [P, P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                           P, C.c_int]),
+        "ivr_volume_luts": (C.c_int, [P, P, C.c_int, P, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)]),
         "ivr_frame_create": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
         "ivr_frame_destroy": (C.c_int, [P, P]),
         "ivr_frame_download": (C.c_int, [P, P, P, P]),
@@ -145,6 +151,8 @@ def _load():
         "ivr_now": (C.c_double, []),
         "ivr_state_pack": (None, [C.POINTER(State), P]),
         "ivr_state_unpack": (None, [P, C.POINTER(State)]),
+        "ivr_state_swap_floats": (None, [C.POINTER(State)]),
+        "ivr_tiff_encode": (C.c_longlong, [P, C.c_int, C.c_int, C.c_int, P, C.c_size_t]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -313,6 +321,24 @@ class Volume:
         check(lib.ivr_volume_gradient(self.ctx.h, self.h, _ptr(out), 0))
         return out
 
+    def histogram(self, bins=True):
+        """(vmin, vmax, hsize, counts or None): the reference's histogram of the values"""
+        lo, hi, hs = C.c_float(), C.c_float(), C.c_int()
+        check(lib.ivr_volume_histogram(self.ctx.h, self.h, C.byref(lo), C.byref(hi), C.byref(hs), None, 0))
+        counts = None
+        if bins:
+            counts = np.zeros(hs.value, np.int32)
+            check(lib.ivr_volume_histogram(self.ctx.h, self.h, C.byref(lo), C.byref(hi), C.byref(hs),
+                                           _ptr(counts), hs.value))
+        return lo.value, hi.value, hs.value, counts
+
+    def luts(self, colormap_id, tf):
+        """initvolluts: colormap over the value range into tf; returns (lutoffset, maxlutindx, hsize)"""
+        off, mx, hs = C.c_int(), C.c_int(), C.c_int()
+        check(lib.ivr_volume_luts(self.ctx.h, self.h, colormap_id, tf.h, C.byref(off), C.byref(mx), C.byref(hs)))
+        tf.n = hs.value
+        return off.value, mx.value, hs.value
+
     def info(self):
         dims = (C.c_int * 3)()
         nb = (C.c_int * 3)()
@@ -341,6 +367,17 @@ class TF:
         a = np.ascontiguousarray(lut, dtype=np.float32).reshape(-1, 4)
         check(lib.ivr_tf_update(self.ctx.h, self.h, _ptr(a), a.shape[0]))
         self.n = a.shape[0]
+
+    def colormap(self, cid, hsize, hmin, hmax):
+        """the reference colormap builder `cid` of hsize entries, on the device"""
+        check(lib.ivr_tf_colormap(self.ctx.h, self.h, cid, hsize, hmin, hmax))
+        self.n = hsize
+
+    def download(self, n=None):
+        n = self.n if n is None else n
+        out = np.zeros((n, 4), np.float32)
+        check(lib.ivr_tf_download(self.ctx.h, self.h, _ptr(out), n))
+        return out
 
     def close(self):
         if self.h:
@@ -516,6 +553,17 @@ def state_unpack(data):
     st = State()
     lib.ivr_state_unpack(_ptr(buf), C.byref(st))
     return st
+
+
+def tiff_encode(rgba32, planes=3):
+    """writetotiff's bytes (utils.c:18-154) of an (h, w, 4) float32 image"""
+    img = np.ascontiguousarray(rgba32, np.float32)
+    h, w = img.shape[:2]
+    out = np.zeros(512 + w * h * planes, np.uint8)
+    n = lib.ivr_tiff_encode(_ptr(img), w, h, planes, _ptr(out), out.size)
+    if n < 0:
+        raise IvrError("tiff_encode: bad image or planes")
+    return out.tobytes()
 
 
 class Batcher:

@@ -10,7 +10,8 @@
  * k_ops on the library's stream, computing exactly what the reference
  * function computes (same float order); ivr_compat_createlocalimage runs the
  * whole pixel loop of createlocalimage (main.c:983-1009) as one launch of the
- * compat-mode frame kernel.
+ * compat-mode frame kernel; the colormap builders (colormap.c:21-181, exported
+ * by compat_cmap.c) run as one launch over the caller's table.
  */
 #include <algorithm>
 #include <cstring>
@@ -77,8 +78,8 @@ int ensure_ctx()
 }
 
 /* mirror the host volume (volume.data, volume.localdims) on the device and
- * size the LUT as the reference's histogram does: (int)(max - min + 1)
- * (mathutil.c:444-470) */
+ * size the LUT as the reference's histogram does, (int)(max - min + 1)
+ * (mathutil.c:444-470), from the device min and max of the uploaded copy */
 int ensure_volume()
 {
     int rc = ensure_ctx();
@@ -93,13 +94,10 @@ int ensure_volume()
     while (brick < 256 && brick < md - 1) brick <<= 1;
     if ((rc = ivr_volume_create(g.ctx, d[0], d[1], d[2], IVR_F32, brick, &g.vol))) return rc;
     if ((rc = ivr_volume_upload(g.ctx, g.vol, volume.data))) return rc;
-    const size_t n = (size_t)d[0] * d[1] * d[2];
-    float lo = volume.data[0], hi = volume.data[0];
-    for (size_t i = 1; i < n; i++) {
-        lo = std::min(lo, volume.data[i]);
-        hi = std::max(hi, volume.data[i]);
-    }
-    g.hsize = std::max(1, std::min((int)((hi - lo) + 1), IVR_MAX_LUT));
+    float lo, hi;
+    int hs;
+    if ((rc = ivr_volume_histogram(g.ctx, g.vol, &lo, &hi, &hs, nullptr, 0))) return rc;
+    g.hsize = std::max(1, std::min(hs, IVR_MAX_LUT));
     g.vol_src = volume.data;
     memcpy(g.dims, d, sizeof d);
     return IVR_OK;
@@ -539,6 +537,20 @@ int ivr_compat_createlocalimage(int rank)
     d.maxlutindx = maxlutindx;
     if ((rc = ivr_render(g.ctx, g.vol, g.tf, &d, g.fr))) return rc;
     return ivr_frame_download(g.ctx, g.fr, nullptr, (float *)colorbuf);
+}
+
+int ivr_compat_colormap(int id, ivr_color4f *l, int hsize, float hmin, float hmax)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (!l || hsize < 1) return ivr_fail(IVR_ERR_ARG, "colormap of %d entries into %p", hsize, (void *)l);
+    int rc = ensure_ctx();
+    if (rc) return rc;
+    if ((rc = io_buffer((size_t)hsize * 4))) return rc;
+    IVR_HIP(hipMemcpyAsync(g.d_io, l, (size_t)hsize * 16, hipMemcpyHostToDevice, g.ctx->stream));
+    if ((rc = ivr_colormap_apply(g.ctx->stream, (float4 *)g.d_io, id, hsize, hmin, hmax))) return rc;
+    IVR_HIP(hipMemcpyAsync(l, g.d_io, (size_t)hsize * 16, hipMemcpyDeviceToHost, g.ctx->stream));
+    IVR_HIP(hipStreamSynchronize(g.ctx->stream));
+    return IVR_OK;
 }
 
 int ivr_compat_compositetiles(float *tilebuf, float *rcolorbuf, int num, int w, int h)

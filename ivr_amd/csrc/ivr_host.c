@@ -263,6 +263,80 @@ void ivr_state_unpack(const unsigned char in[184], ivr_state *st)
     st->winheight = (int)be32(v[2]);
 }
 
+static float bswap_f(float f)
+{
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    u = (u >> 24) | ((u >> 8) & 0xff00u) | ((u << 8) & 0xff0000u) | (u << 24);
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+void ivr_state_swap_floats(ivr_state *st)
+{
+    st->alpha = bswap_f(st->alpha);
+    for (int i = 0; i < 6; i++) st->newconst[i] = bswap_f(st->newconst[i]);
+    for (int i = 0; i < 4; i++) st->vv[i] = bswap_f(st->vv[i]);
+    for (int i = 0; i < 16; i++) st->mvm[i] = bswap_f(st->mvm[i]);
+    for (int i = 0; i < 16; i++) st->pm[i] = bswap_f(st->pm[i]);
+}
+
+/* ---------------- TIFF image output ---------------- */
+
+/* one 12-byte directory entry, big-endian (make_tag, utils.c:135-154) */
+static unsigned char *tiff_tag(unsigned char *b, int tag, int type, long lng, long fld, int *cnt)
+{
+    const unsigned long v[4] = {(unsigned long)tag, (unsigned long)type, (unsigned long)lng, (unsigned long)fld};
+    const int width[4] = {2, 2, 4, 4};
+    for (int k = 0; k < 4; k++)
+        for (int byte = width[k] - 1; byte >= 0; byte--) *b++ = (unsigned char)((v[k] >> (8 * byte)) & 0xffu);
+    (*cnt)++;
+    return b;
+}
+
+long long ivr_tiff_encode(const float *rgba, int w, int h, int planes, unsigned char *out, size_t cap)
+{
+    if (!rgba || !out || w < 1 || h < 1 || w > 65535 || h > 65535 || (planes != 1 && planes != 3)) return -1;
+    const size_t need = 512 + (size_t)w * h * planes;
+    if (cap < need) return -1;
+    /* header (writetiffheader, utils.c:24-100): "MM", 42, the directory at
+     * byte 10 with its entry count at byte 11, SHORT values in the upper
+     * half of their field; 8 bits per sample written at byte 256 for RGB */
+    unsigned char hdr[512];
+    memset(hdr, 0, sizeof hdr);
+    hdr[0] = 0x4d;
+    hdr[1] = 0x4d;
+    hdr[3] = 42;
+    hdr[7] = 10;
+    int cnt = 0;
+    const int SHORT_T = 3, LONG_T = 4;
+    unsigned char *b = hdr + 12;
+    b = tiff_tag(b, 254, LONG_T, 1, 0, &cnt);
+    b = tiff_tag(b, 256, SHORT_T, 1, (long)w << 16, &cnt);
+    b = tiff_tag(b, 257, SHORT_T, 1, (long)h << 16, &cnt);
+    b = planes == 3 ? tiff_tag(b, 258, SHORT_T, 3, 256, &cnt) : tiff_tag(b, 258, SHORT_T, 1, 8L << 16, &cnt);
+    b = tiff_tag(b, 259, SHORT_T, 1, 1L << 16, &cnt);
+    b = tiff_tag(b, 262, SHORT_T, 1, (planes == 3 ? 2L : 1L) << 16, &cnt);
+    b = tiff_tag(b, 273, LONG_T, 1, 512, &cnt);
+    b = tiff_tag(b, 277, SHORT_T, 1, (long)planes << 16, &cnt);
+    b = tiff_tag(b, 278, SHORT_T, 1, (long)h << 16, &cnt);
+    b = tiff_tag(b, 279, LONG_T, 1, (long)h * w * planes, &cnt);
+    b = tiff_tag(b, 284, SHORT_T, 1, 1L << 16, &cnt);
+    b = tiff_tag(b, 0, 0, 0, 0, &cnt);
+    hdr[11] = (unsigned char)(cnt - 1);
+    if (planes == 3) hdr[257] = hdr[259] = hdr[261] = 8;
+    memcpy(out, hdr, 512);
+    /* samples (writetiffdata, utils.c:102-133): rows in the image's order,
+     * each channel value * 255 truncated to a byte */
+    unsigned char *d = out + 512;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            const float *px = rgba + ((size_t)y * w + x) * 4;
+            for (int c = 0; c < planes; c++) *d++ = (unsigned char)(px[c] * 255);
+        }
+    return (long long)need;
+}
+
 /* ---------------- transfer-function builder ---------------- */
 
 static float clamp01(float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); }

@@ -221,6 +221,9 @@ int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles);
 /* n u64 words set to zero by a kernel (no memset node, so callers may capture it) */
 int ivr_launch_zero_u64(hipStream_t stream, unsigned long long *p, size_t n);
 
+/* colormap builder `id` (luts.hip) over the current n entries of a device LUT */
+int ivr_colormap_apply(hipStream_t st, float4 *d_lut, int id, int n, float hmin, float hmax);
+
 /* volume kernels (volume.hip) */
 int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *vol);
 

@@ -23,6 +23,32 @@
  * u64 id, i32 status, i32 width, i32 height, then the RGBA8 bytes (none when
  * status != 0).  All extension integers are little-endian.
  *
+ * Compat serving (--compat COLORMAP XFER, or the reference server's own
+ * command line, below): every request is rendered as the reference render
+ * server renders it (main.c:206-250): ray.c semantics (IVR_MODE_COMPAT),
+ * transfer function XFER, the LUT built as initvolluts builds it
+ * (main.c:1075-1200: the volume's histogram on the device, colormap
+ * COLORMAP over its [min, max], lutoffset = (int)(min + 0.5), maxlutindx =
+ * hsize - 1), one STATE at a time; the reply is the RGBA8 of
+ * createlocalimage's colour buffer.  An extension message's LUT, when given,
+ * replaces the volume's for that request.
+ *
+ * The reference server's command line is accepted as is:
+ *   ivrserver FILE W H D ENDIAN COLORMAP XFER PX PY PZ TILEW TILEH ANGLE AX AY AZ
+ * (parseargs, main.c:1047-1073; ENDIAN 1 little, 0 big as initvolattrs
+ * compares it, main.c:386-392): compat serving of the raw volume FILE behind
+ * the relay at localhost:1400 (main.c:19-23), or at $IVR_RELAY (HOST:PORT);
+ * TILEW x TILEH rounds each window down to whole tiles as setviewport does
+ * (mathutil.h:147-163) and that viewport is rendered, as the reference
+ * server renders it (see render_compat); the decomposition and view arguments have no
+ * effect (one GPU holds the whole volume; the client's STATE sets the
+ * view).  --tile TW TH sets the rounding in the named form (default 1 1).
+ *
+ * --swap-floats byte-swaps the STATE's floats on receipt, as the reference's
+ * BIGENDIAN build does (connection.c:191-197), for peers of the other byte
+ * order.  --tiff DIR (compat) also writes each frame's colour buffer as
+ * DIR/frameNNNNN.tiff in writetotiff's format (utils.c:18-154).
+ *
  * Volumes (--volume, repeatable):
  *   gen:C:N[:f16]               synthetic generator of benchmark config C at N^3
  *   raw:PATH:NX:NY:NZ:E[:f16]   raw float32 file, x fastest, E = 1 little /
@@ -55,6 +81,9 @@ typedef struct {
     float lut[256 * 4];
     float lutscale;
     int lutoffset;
+    /* compat serving: the initvolluts LUT on the device */
+    ivr_tf *tf;
+    int maxlutindx, hsize;
 } served_volume;
 
 typedef struct {
@@ -77,6 +106,16 @@ static int g_nvol = 0;
 static client g_cl[MAX_CLIENTS];
 static int g_skip = 1;
 static unsigned long long g_next_rid = 1;
+/* compat serving */
+static int g_compat = 0, g_cmap = 0, g_xfer = 3;
+static int g_tile_w = 1, g_tile_h = 1; /* compat: the viewport is a whole number of tiles */
+static int g_swap_floats = 0;           /* STATE floats of the other byte order (BIGENDIAN) */
+static const char *g_tiff_dir = NULL;   /* compat: each frame also written as a TIFF */
+static int g_tiff_count = 0;
+static ivr_ctx *g_ctx = NULL;
+static ivr_frame *g_frame = NULL; /* compat frames, reused while the size stays */
+static int g_fw = 0, g_fh = 0;
+static ivr_tf *g_req_tf = NULL;    /* an extension message's own LUT */
 
 static int send_all(int fd, const void *p, size_t n)
 {
@@ -188,8 +227,92 @@ static int add_volume(ivr_ctx *ctx, const char *spec)
     } else {
         return -1;
     }
+    if (g_compat) {
+        /* initvolluts: the histogram's range sizes the LUT, colormap over it */
+        const float one[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if ((rc = ivr_tf_create(ctx, one, 1, &sv->tf)) ||
+            (rc = ivr_volume_luts(ctx, sv->vol, g_cmap, sv->tf, &sv->lutoffset, &sv->maxlutindx, &sv->hsize)))
+            return rc;
+        fprintf(stderr, "ivrserver: LUT of %d entries (colormap %d), lutoffset %d\n", sv->hsize, g_cmap,
+                sv->lutoffset);
+    }
     g_nvol++;
     return g_nvol - 1;
+}
+
+/* the compat frame's colour buffer as DIR/frameNNNNN.tiff (writetotiff, utils.c:18-154, RGB) */
+static int write_tiff(int w, int h)
+{
+    const size_t np = (size_t)w * h, bytes = 512 + np * 3;
+    float *img = (float *)malloc(np * 16);
+    unsigned char *out = (unsigned char *)malloc(bytes);
+    int rc = (!img || !out) ? IVR_ERR_NOMEM : ivr_frame_download(g_ctx, g_frame, NULL, img);
+    if (!rc && ivr_tiff_encode(img, w, h, 3, out, bytes) < 0) rc = IVR_ERR_ARG;
+    if (!rc) {
+        char path[1024];
+        snprintf(path, sizeof path, "%s/frame%05d.tiff", g_tiff_dir, g_tiff_count++);
+        FILE *f = fopen(path, "wb");
+        if (!f || fwrite(out, 1, bytes, f) != bytes) rc = IVR_ERR_IO;
+        if (f) fclose(f);
+    }
+    free(img);
+    free(out);
+    return rc;
+}
+
+/* One request rendered as the reference render server renders it (compat)
+ * into the window's winwidth x winheight RGBA8 bytes.  setviewport
+ * (mathutil.h:147-163) rounds the window down to whole tiles and the
+ * reference renders and sends that viewport (writetomemory, main.c:1340-1410);
+ * for a window of whole tiles these are the window's bytes.  For other
+ * windows the reference sends fewer bytes than the relay forwards
+ * (myserver.c:147-153 reads winwidth x winheight x 4) and the session stalls;
+ * here the viewport's pixels fill the bottom left of the window and the rest
+ * is zero, so the protocol keeps going. */
+static int render_compat(int vol, const ivr_state *win, const float *lut, int nl, unsigned char *rgba8)
+{
+    served_volume *sv = &g_vol[vol];
+    const int W = win->winwidth, H = win->winheight;
+    ivr_state vst = *win;
+    vst.winwidth = W / g_tile_w * g_tile_w;
+    vst.winheight = H / g_tile_h * g_tile_h;
+    const ivr_state *st = &vst;
+    const int padded = vst.winwidth != W || vst.winheight != H;
+    memset(rgba8, 0, (size_t)W * H * 4);
+    if (vst.winwidth == 0 || vst.winheight == 0) return 0;
+    int rc;
+    if (!g_frame || g_fw != st->winwidth || g_fh != st->winheight) {
+        if (g_frame) ivr_frame_destroy(g_ctx, g_frame);
+        g_frame = NULL;
+        if ((rc = ivr_frame_create(g_ctx, st->winwidth, st->winheight, g_tiff_dir != NULL, &g_frame))) return rc;
+        g_fw = st->winwidth;
+        g_fh = st->winheight;
+    }
+    ivr_tf *tf = sv->tf;
+    ivr_frame_desc d;
+    ivr_desc_config(&d);
+    d.state = *st;
+    d.mode = IVR_MODE_COMPAT;
+    d.xfer = g_xfer;
+    d.lutscale = 1.0f;
+    d.lutoffset = sv->lutoffset;
+    d.maxlutindx = sv->maxlutindx;
+    if (lut && nl > 0) {
+        if ((rc = g_req_tf ? ivr_tf_update(g_ctx, g_req_tf, lut, nl) : ivr_tf_create(g_ctx, lut, nl, &g_req_tf)))
+            return rc;
+        tf = g_req_tf;
+        d.maxlutindx = nl - 1;
+    }
+    if ((rc = ivr_render(g_ctx, sv->vol, tf, &d, g_frame))) return rc;
+    if (g_tiff_dir && (rc = write_tiff(st->winwidth, st->winheight))) return rc;
+    if (!padded) return ivr_frame_download(g_ctx, g_frame, rgba8, NULL);
+    const size_t row = (size_t)vst.winwidth * 4;
+    unsigned char *view = (unsigned char *)malloc(row * vst.winheight);
+    if (!view) return IVR_ERR_NOMEM;
+    rc = ivr_frame_download(g_ctx, g_frame, view, NULL);
+    for (int y = 0; y < vst.winheight && !rc; y++) memcpy(rgba8 + (size_t)y * W * 4, view + (size_t)y * row, row);
+    free(view);
+    return rc;
 }
 
 static client *new_client(int fd)
@@ -273,6 +396,7 @@ static int handle(ivr_batcher *bt, client *c, size_t len)
     }
     ivr_state st;
     ivr_state_unpack(m, &st);
+    if (g_swap_floats) ivr_state_swap_floats(&st);
     if (st.finish) return 1;
     c->w = st.winwidth;
     c->h = st.winheight;
@@ -284,6 +408,13 @@ static int handle(ivr_batcher *bt, client *c, size_t len)
         c->img_cap = c->img ? need : 0;
         if (!c->img) return -1;
     }
+    if (g_compat) { /* one STATE at a time, as the reference server loop (main.c:206-250) */
+        const int v = (vol >= 0 && vol < g_nvol) ? vol : -1;
+        int rc = v < 0 ? IVR_ERR_ARG : render_compat(v, &st, lut, nl, c->img);
+        if (rc) fprintf(stderr, "ivrserver: compat frame failed: %s\n", ivr_last_error());
+        return reply(c, rc) ? -1 : 0;
+    }
+    /* the batcher's frames */
     ivr_request r;
     memset(&r, 0, sizeof r);
     r.id = c->rid = g_next_rid++;
@@ -368,7 +499,9 @@ static int read_client(ivr_batcher *bt, client *c)
 static void usage(void)
 {
     fprintf(stderr, "usage: ivrserver (--listen PORT | --relay HOST:PORT) --volume SPEC [--volume SPEC ...]\n"
-                    "       [--budget-gb G] [--slots N] [--max W H] [--skip 0|1] [--device D]\n");
+                    "       [--budget-gb G] [--slots N] [--max W H] [--skip 0|1] [--device D]\n"
+                    "       [--compat COLORMAP XFER] [--tile TW TH] [--tiff DIR] [--swap-floats]\n"
+                    "   or: ivrserver FILE W H D ENDIAN COLORMAP XFER PX PY PZ TILEW TILEH ANGLE AX AY AZ\n");
 }
 
 int main(int argc, char **argv)
@@ -377,6 +510,22 @@ int main(int argc, char **argv)
     const char *specs[MAX_VOLS];
     int nspec = 0, slots = 16, maxw = 1920, maxh = 1080, device = 0;
     double budget_gb = 64.0;
+    char refspec[600], relay_buf[256];
+    if (argc == 17 && argv[1][0] != '-') {
+        /* the reference server's command line (parseargs, main.c:1047-1073) */
+        const int endian = atoi(argv[5]);
+        snprintf(refspec, sizeof refspec, "raw:%s:%s:%s:%s:%d", argv[1], argv[2], argv[3], argv[4], endian ? 1 : 0);
+        specs[nspec++] = refspec;
+        g_compat = 1;
+        g_cmap = atoi(argv[6]);
+        g_xfer = atoi(argv[7]);
+        g_tile_w = atoi(argv[11]);
+        g_tile_h = atoi(argv[12]);
+        const char *env = getenv("IVR_RELAY");
+        snprintf(relay_buf, sizeof relay_buf, "%s", env && *env ? env : "localhost:1400");
+        relay = relay_buf;
+        argc = 1; /* nothing else to parse */
+    }
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--listen") && i + 1 < argc) listen_port = argv[++i];
         else if (!strcmp(argv[i], "--relay") && i + 1 < argc) relay = argv[++i];
@@ -386,6 +535,16 @@ int main(int argc, char **argv)
         else if (!strcmp(argv[i], "--max") && i + 2 < argc) { maxw = atoi(argv[++i]); maxh = atoi(argv[++i]); }
         else if (!strcmp(argv[i], "--skip") && i + 1 < argc) g_skip = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) device = atoi(argv[++i]);
+        else if (!strcmp(argv[i], "--swap-floats")) g_swap_floats = 1;
+        else if (!strcmp(argv[i], "--tiff") && i + 1 < argc) g_tiff_dir = argv[++i];
+        else if (!strcmp(argv[i], "--tile") && i + 2 < argc) {
+            g_tile_w = atoi(argv[++i]);
+            g_tile_h = atoi(argv[++i]);
+        } else if (!strcmp(argv[i], "--compat") && i + 2 < argc) {
+            g_compat = 1;
+            g_cmap = atoi(argv[++i]);
+            g_xfer = atoi(argv[++i]);
+        }
         else { usage(); return 2; }
     }
     if ((!listen_port && !relay) || nspec == 0) { usage(); return 2; }
@@ -393,6 +552,16 @@ int main(int argc, char **argv)
     ivr_ctx *ctx;
     ivr_batcher *bt;
     if (ivr_ctx_create(device, &ctx)) { fprintf(stderr, "ivrserver: %s\n", ivr_last_error()); return 1; }
+    g_ctx = ctx;
+    if (g_tile_w < 1 || g_tile_h < 1) {
+        fprintf(stderr, "ivrserver: tile sizes must be positive\n");
+        return 2;
+    }
+    if (g_compat && (g_cmap < 0 || g_cmap > 4 || g_xfer < 0 || g_xfer > 5)) {
+        /* the reference builds no LUT for colormap -1 (main.c:340) and renders with none */
+        fprintf(stderr, "ivrserver: compat serving needs colormap 0..4 and transfer function 0..5\n");
+        return 2;
+    }
     if (ivr_batcher_create(ctx, (unsigned long long)(budget_gb * 1073741824.0), slots, maxw, maxh, &bt)) {
         fprintf(stderr, "ivrserver: %s\n", ivr_last_error());
         return 1;
@@ -474,7 +643,12 @@ int main(int argc, char **argv)
         if (work) usleep(100);
     }
     ivr_batcher_destroy(bt);
-    for (int v = 0; v < g_nvol; v++) ivr_volume_destroy(ctx, g_vol[v].vol);
+    if (g_frame) ivr_frame_destroy(ctx, g_frame);
+    if (g_req_tf) ivr_tf_destroy(ctx, g_req_tf);
+    for (int v = 0; v < g_nvol; v++) {
+        if (g_vol[v].tf) ivr_tf_destroy(ctx, g_vol[v].tf);
+        ivr_volume_destroy(ctx, g_vol[v].vol);
+    }
     ivr_ctx_destroy(ctx);
     return 0;
 }

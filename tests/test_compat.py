@@ -54,7 +54,10 @@ def _load_refgpu():
     return lib
 
 
+@pytest.mark.gpu
 def test_colormaps_through_dropin():
+    """the drop-in builders (one device launch over the caller's table) give
+    the reference's colormaps bit for bit"""
     lib = _load_refgpu()
     z = np.load(os.path.join(G, "colormaps.npz"))
     for key in z.files:

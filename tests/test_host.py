@@ -10,6 +10,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 import ivr_amd as ivr
 from ivr_amd import configs
@@ -89,3 +90,42 @@ def test_orbit_state_is_the_clients_modelview():
 def test_state_layout():
     assert C.sizeof(ivr.State) == 184
     assert ivr.State.mvm.offset == 56 and ivr.State.pm.offset == 120
+
+
+def test_state_float_swap_is_recvstate_under_bigendian():
+    """a STATE whose floats arrive in the other byte order (a BIGENDIAN
+    build's peer, connection.c:191-197) unpacks to the sender's STATE after
+    ivr_state_swap_floats; swapping twice is the identity"""
+    st = ivr.orbit_state(77, 320, 200)
+    st.alpha = 0.625
+    st.newconst[2] = -0.25
+    wire = bytearray(ivr.state_pack(st))
+    for off in range(12, 184, 4):  # every float of the wire message, byte-reversed
+        wire[off:off + 4] = wire[off:off + 4][::-1]
+    got = ivr.state_unpack(bytes(wire))
+    assert bytes(got) != bytes(st)
+    ivr.lib.ivr_state_swap_floats(C.byref(got))
+    assert bytes(got) == bytes(st)
+    ivr.lib.ivr_state_swap_floats(C.byref(got))
+    ivr.lib.ivr_state_swap_floats(C.byref(got))
+    assert bytes(got) == bytes(st)
+
+
+@pytest.mark.parametrize("planes", [1, 3])
+def test_tiff_bytes_match_the_reference_writetotiff(planes):
+    """ivr_tiff_encode writes the bytes of the reference's writetotiff
+    (utils.c:18-154, compiled from its sources) for the same RGBA32F image"""
+    import oracle
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("the compiled reference (oracle/_ref) is not built")
+    rng = np.random.default_rng(planes)
+    h, w = 37, 53
+    img = rng.uniform(0.0, 1.0, (h, w, 4)).astype(np.float32)
+    img[0, 0] = [0.0, 1.0, 0.5, 1.0]
+    want = np.zeros(512 + w * h * planes, np.uint8)
+    ref.writetotiff(want.ctypes.data_as(C.c_char_p), C.c_long(w), C.c_long(h),
+                    C.c_long(planes), img.ctypes.data_as(C.POINTER(C.c_float)))
+    got = ivr.tiff_encode(img, planes)
+    assert got[:4] == b"MM\x00*" and len(got) == want.size
+    assert got == want.tobytes()

@@ -3,6 +3,7 @@ protocol: concurrent clients in --listen mode (plain STATE and the versioned
 extension) and the drop-in --relay mode behind a relay that behaves like the
 reference's myserver (myserver.c:104-165).  Every returned frame equals the
 in-process render of the same request."""
+import ctypes as C
 import os
 import socket
 import struct
@@ -301,4 +302,85 @@ def test_requests_sent_ahead_are_served(ctx, refs):
     finally:
         ls.close()
         if srv.poll() is None:
+            srv.kill()
+
+
+@pytest.mark.skipif(not os.path.exists(RELAY), reason="reference relay not built (make -C oracle relay)")
+@pytest.mark.parametrize("cmap,xfer", [(4, 2), (2, 3)])
+def test_reference_command_line_behind_the_reference_relay(ctx, tmp_path, cmap, xfer):
+    """ivrserver started with the reference server's own command line (raw
+    volume, colormap, transfer function, decomposition and view arguments)
+    behind the reference relay compiled from myserver.c: every frame is
+    bit-identical to the compiled reference's createlocalimage for the same
+    STATE and the LUT its initvolluts builds (maxlutindx = hsize - 1 as the
+    MPI render server sets it), quantised as main.c:1404 does"""
+    import oracle
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("the compiled reference (oracle/_ref) is not built")
+    nx, ny, nz = 61, 47, 53
+    blobs = ivr.generate_host(configs.gen_params(1), nx, ny, nz)
+    host = (np.float32(1468.0) + np.float32(3518.0) * blobs).astype(np.float32)
+    path = str(tmp_path / "shell.raw")
+    host.tofile(path)  # little endian: ENDIAN 1
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    tile = 8
+    assert ref.ivrref_init(path.encode(), nx, ny, nz, cmap, xfer, tile) == 0
+    hs = int(np.float32(np.float32(host.max() - host.min()) + np.float32(1.0)))
+    lut = np.zeros((hs, 4), np.float32)
+    ref.ivrref_get_lut(f(lut), hs)
+    ref.ivrref_set_lut(f(lut), hs, ref.ivrref_lutoffset(), hs - 1)
+    ref.ivrref_set_xfer(xfer)
+    port = free_port()
+    relay = subprocess.Popen([RELAY, str(port)], stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    srv = None
+    log_path = str(tmp_path / "server.log")
+    try:
+        cl = None
+        t0 = time.time()
+        while time.time() - t0 < 30:
+            try:
+                cl = socket.create_connection(("127.0.0.1", port), timeout=30)
+                break
+            except OSError:
+                time.sleep(0.1)
+        assert cl is not None, "relay did not listen"
+        assert recv_all(cl, 13) == b"Hello, world!"
+        cl.sendall(b"Hello, server!")
+        env = dict(os.environ, IVR_RELAY="127.0.0.1:%d" % port)
+        args = [path, str(nx), str(ny), str(nz), "1", str(cmap), str(xfer), "1", "1", "1", str(tile), str(tile),
+                "180", "1", "0", "0"]
+        with open(log_path, "w") as log:
+            srv = subprocess.Popen([SERVER] + args, stderr=log, env=env)
+        # the last window is not a whole number of tiles: both render the viewport setviewport
+        # rounds it to (196x100 -> 192x96); the reference would send only its bytes, fewer than
+        # the relay forwards, and stall; ivrserver sends the window with the viewport at its
+        # bottom left and zeros beyond
+        for seed, (w, h) in ((3401, (160, 120)), (3402, (128, 128)), (3403, (196, 100))):
+            st = ivr.orbit_state(seed, w, h)
+            vw, vh = w // tile * tile, h // tile * tile
+            cl.sendall(ivr.state_pack(st))
+            win = np.frombuffer(recv_all(cl, w * h * 4), np.uint8).reshape(h, w, 4)
+            img = win[:vh, :vw]
+            assert not win[vh:].any() and not win[:, vw:].any()
+            out = np.zeros((h, w, 4), np.float32)
+            rect = (C.c_int * 4)()
+            ref.ivrref_render(f(np.array(st.mvm, np.float32)), f(np.array(st.pm, np.float32)),
+                              f(np.array(st.vv, np.float32)), f(np.array(st.newconst, np.float32)),
+                              C.c_float(st.alpha), w, h, f(out), rect)
+            want = oracle.quantize(out.reshape(-1, 4)[:vw * vh].reshape(vh, vw, 4))
+            assert (want[..., 3] > 0).mean() > 0.05
+            assert np.array_equal(img, want), (seed, int(np.abs(img.astype(int) - want.astype(int)).max()))
+        fin = ivr.State()
+        fin.finish = 1
+        cl.sendall(ivr.state_pack(fin))
+        assert srv.wait(timeout=60) == 0
+        cl.close()
+    except Exception:
+        print("server log:\n" + open(log_path).read(), flush=True)
+        raise
+    finally:
+        relay.kill()
+        relay.wait(timeout=30)
+        if srv is not None and srv.poll() is None:
             srv.kill()
