@@ -745,6 +745,54 @@ def cpu_baseline(vol, st, lut, gpu_img, W, H, comp, brick, target_s):
     }
 
 
+def _load_refgpu():
+    """the reference's own main.c/mathutil.c/utils.c linked against
+    libivr_compat.so (oracle/_ref/libivrref_gpu.so), or None"""
+    import ctypes as C
+    path = os.path.join(ROOT, "oracle", "_ref", "libivrref_gpu.so")
+    return C.CDLL(path) if os.path.exists(path) else None
+
+
+def dropin_per_ray(refgpu, lut, st, W, H, want):
+    """The unchanged reference host relinked against the drop-in: its own
+    createlocalimage pixel loop (OpenMP over rows, main.c:983-1009) calling
+    the drop-in raycast per pixel (calls of concurrent threads combined into
+    shared launches), and the same host with the loop replaced by
+    ivr_compat_createlocalimage (one launch); ms/frame of each and whether
+    their images equal the reference's"""
+    import ctypes as C
+    import numpy as np
+    import ivr_amd as ivr
+    f = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))  # noqa: E731
+    refgpu.ivrref_set_lut(f(lut), lut.shape[0], 0, 255)
+    refgpu.ivrref_set_xfer(ivr.CHEM)
+    args = (f(np.array(st.mvm, np.float32)), f(np.array(st.pm, np.float32)), f(np.array(st.vv, np.float32)),
+            f(np.array(st.newconst, np.float32)), C.c_float(st.alpha), W, H)
+    res = {}
+    for name, gpu_frame in (("per_pixel_raycast", 0), ("createlocalimage_one_launch", 1)):
+        refgpu.ivrref_frame_on_gpu(gpu_frame)
+        out = np.zeros((H, W, 4), np.float32)
+        rect = (C.c_int * 4)()
+        l0, r0 = C.c_ulonglong(), C.c_ulonglong()
+        refgpu.ivr_compat_raycast_stats(C.byref(l0), C.byref(r0))
+        with stdout_to_stderr():
+            refgpu.ivrref_render(*args, f(out), rect)  # warm-up: volume and LUT mirrored to the device
+            t = time.perf_counter()
+            refgpu.ivrref_render(*args, f(out), rect)
+            ms = (time.perf_counter() - t) * 1e3
+        l1, r1 = C.c_ulonglong(), C.c_ulonglong()
+        refgpu.ivr_compat_raycast_stats(C.byref(l1), C.byref(r1))
+        e = {"ms_per_frame": round(ms, 2),
+             "images_identical": bool(np.array_equal(out.view(np.uint32), want.view(np.uint32)))}
+        if gpu_frame == 0:
+            rays, launches = (r1.value - r0.value) // 2, (l1.value - l0.value) // 2
+            e.update(rays=int(rays), launches=int(launches), rays_per_launch=round(rays / max(1, launches), 2))
+        res[name] = e
+        log("drop-in %s: %.2f ms/frame %s" % (name, ms, e))
+    refgpu.ivrref_frame_on_gpu(0)
+    return res
+
+
 class stdout_to_stderr:
     """route file descriptor 1 to 2 (the reference prints progress to stdout)"""
 
@@ -806,6 +854,9 @@ def reference_cpu(ctx, vol, st, lut, W, H):
         host.tofile(path)
         with stdout_to_stderr():
             rc = ref.ivrref_init(path.encode(), nx, ny, nz, -1, 3, 8)
+            refgpu = _load_refgpu()
+            if refgpu is not None and refgpu.ivrref_init(path.encode(), nx, ny, nz, -1, 3, 8) != 0:
+                refgpu = None
     finally:
         os.unlink(path)
     del host
@@ -830,9 +881,11 @@ def reference_cpu(ctx, vol, st, lut, W, H):
             timed[used] = time.perf_counter() - t
         if nt == threads:
             same = bool(np.array_equal(out.view(np.uint32), g32.view(np.uint32)))
+    ref.ivrref_set_threads(threads)  # OpenMP's thread count is shared by every library of the process
     c_s = timed[max(timed)]
     log("reference ray.c: %s for %d samples; GPU compat %.3f ms; images identical: %s"
         % (", ".join("%.2fs on %d threads" % (v, k) for k, v in sorted(timed.items())), S, g_ms, same))
+    dropin = None if refgpu is None else dropin_per_ray(refgpu, lutc, st, W, H, out)
     return {
         "value": round(S / c_s / 1e9, 5),
         "unit": "Gsamples/s",
@@ -847,6 +900,7 @@ def reference_cpu(ctx, vol, st, lut, W, H):
         "gpu_same_path_ms": round(g_ms, 4),
         "gpu_over_reference": round((S / g_ms / 1e6) / (S / c_s / 1e9), 1),
         "images_identical": same,
+        "dropin_relinked_host": dropin,
     }
 
 

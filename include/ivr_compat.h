@@ -11,8 +11,10 @@
  * defines (volume, aabb, lut, lutoffset, maxlutindx, mystate, cnt,
  * colorbuf, viewport, viewvolume, mvm, pm; main.c:63-105).
  *
- * The per-ray and per-sample functions run on the GPU (one launch per call,
- * bit-exact with the reference); the frame loop of createlocalimage
+ * The per-ray and per-sample functions run on the GPU, bit-exact with the
+ * reference: one launch per call, except that raycast calls made at the same
+ * time by several threads (the reference's OpenMP pixel loop) share a launch;
+ * the frame loop of createlocalimage
  * (main.c:937-1026) becomes one launch through
  * ivr_compat_createlocalimage(rank), which a host replaces its pixel loop
  * with.  The LUT builders of colormap.c run on the device over the caller's
@@ -115,6 +117,8 @@ int ivr_compat_compositetiles(float *tilebuf, float *rcolorbuf, int num, int w, 
  * write it (fields they leave alone keep the caller's values); the exported
  * builders above call it */
 int ivr_compat_colormap(int id, ivr_color4f *lut, int hsize, float hmin, float hmax);
+/* launches of the combined raycast calls so far and the rays they cast */
+void ivr_compat_raycast_stats(unsigned long long *launches, unsigned long long *rays);
 /* forget the device mirror of the host volume */
 void ivr_compat_invalidate(void);
 

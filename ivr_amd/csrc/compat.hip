@@ -8,12 +8,14 @@
  * function pointers of ray.h:25 and transferfuncs.h:22) and reads the host's
  * globals (main.c:63-105).  Every per-ray or per-sample call is one launch of
  * k_ops on the library's stream, computing exactly what the reference
- * function computes (same float order); ivr_compat_createlocalimage runs the
+ * function computes (same float order), raycast calls of concurrent threads
+ * combined into one launch; ivr_compat_createlocalimage runs the
  * whole pixel loop of createlocalimage (main.c:983-1009) as one launch of the
  * compat-mode frame kernel; the colormap builders (colormap.c:21-181, exported
  * by compat_cmap.c) run as one launch over the caller's table.
  */
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
 #include <mutex>
 #include <vector>
@@ -67,6 +69,14 @@ struct Compat {
     float newconst[6] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
     float *d_io = nullptr;
     size_t io_cap = 0;
+    /* pinned, device-mapped staging of the per-ray ops: the kernel reads its
+     * rays from and writes its results to host memory, so a call is one
+     * launch and one synchronisation, with no copies queued */
+    float *h_io = nullptr, *h_io_dev = nullptr;
+    size_t h_cap = 0;
+    std::vector<ivr_color4f> lut_copy; /* the host LUT last uploaded */
+    const ivr_color4f *lut_src = nullptr;
+    unsigned long long ray_batches = 0, rays_cast = 0; /* combined raycast launches */
 };
 
 Compat g;
@@ -103,12 +113,34 @@ int ensure_volume()
     return IVR_OK;
 }
 
+/* the host LUT to the device, skipped while its address, size and contents
+ * equal those of the last upload (the per-ray calls check it every time) */
 int upload_lut(const ivr_color4f *l, int n)
 {
     if (!l) return ivr_fail(IVR_ERR_ARG, "LUT pointer is NULL");
     n = std::max(1, std::min(n, IVR_MAX_LUT));
-    if (!g.tf) return ivr_tf_create(g.ctx, (const float *)l, n, &g.tf);
-    return ivr_tf_update(g.ctx, g.tf, (const float *)l, n);
+    if (g.tf && g.lut_src == l && (int)g.lut_copy.size() == n && g.tf->n == n &&
+        !memcmp(g.lut_copy.data(), l, sizeof(ivr_color4f) * (size_t)n))
+        return IVR_OK;
+    int rc = g.tf ? ivr_tf_update(g.ctx, g.tf, (const float *)l, n) : ivr_tf_create(g.ctx, (const float *)l, n, &g.tf);
+    if (rc) return rc;
+    g.lut_src = l;
+    g.lut_copy.assign(l, l + n);
+    return IVR_OK;
+}
+
+/* the pinned mapped staging buffer, at least `floats` long */
+int host_io(size_t floats)
+{
+    if (floats <= g.h_cap) return IVR_OK;
+    if (g.h_io) (void)hipHostFree(g.h_io);
+    g.h_io = g.h_io_dev = nullptr;
+    g.h_cap = 0;
+    const size_t cap = std::max<size_t>(floats, 1 << 16);
+    IVR_HIP(hipHostMalloc((void **)&g.h_io, cap * sizeof(float), hipHostMallocMapped | hipHostMallocCoherent));
+    IVR_HIP(hipHostGetDevicePointer((void **)&g.h_io_dev, g.h_io, 0));
+    g.h_cap = cap;
+    return IVR_OK;
 }
 
 int io_buffer(size_t floats)
@@ -297,6 +329,47 @@ __global__ void k_ops(const RenderArgs a, int op, int n, const float *in, float 
     }
 }
 
+/* raycast (ray.c:59-85) of one ray per wavefront: the lanes sample the
+ * ray's (at most 256) positions and fetch their LUT entries in parallel into
+ * LDS, then one lane applies the transfer function over them in the
+ * reference's order (back to front for SEISMIC2, CHEM and RGBBLEND), so a
+ * call waits for one round of gathers instead of 256 dependent ones.  Same
+ * float operations as march_compat. */
+__global__ __launch_bounds__(64) void k_raycast_rays(const RenderArgs a, int n, const float *in, float *out)
+{
+    __shared__ float s_val[256];
+    __shared__ float4 s_ent[256];
+    const int r = blockIdx.x, lane = threadIdx.x;
+    if (r >= n) return;
+    const float *o = in + r * 6, *d = o + 3;
+    float tin, tout;
+    int iend = 0;
+    if (slab(a.dconst, o, d, tin, tout)) {
+        const float dt = 0.001f; /* DELTASAMPLE (types.h:17) */
+        iend = 1.0f + (tout - tin) / dt + 0.5f;
+        iend = iend < 256 ? iend : 256;
+        for (int q = lane; q < iend; q += 64) {
+            const float t = tin + ((float)q * dt);
+            const float v = sample_ref<float>(a, a.boff, d[0] * t + o[0], d[1] * t + o[1], d[2] * t + o[2]);
+            const int idx = tf_lut_index(a, a.xfer, v);
+            s_val[q] = v;
+            s_ent[q] = idx >= 0 ? a.lut[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+    }
+    __syncthreads();
+    if (lane) return;
+    Acc c = {0.0f, 0.0f, 0.0f, 0.0f};
+    const bool backward = tf_backward(a.xfer);
+    for (int q = 0; q < iend; q++) {
+        const int si = backward ? iend - 1 - q : q;
+        tf_apply(a, a.xfer, s_val[si], s_ent[si], c);
+    }
+    out[r * 4 + 0] = c.r;
+    out[r * 4 + 1] = c.g;
+    out[r * 4 + 2] = c.b;
+    out[r * 4 + 3] = c.a;
+}
+
 /* compositetiles (main.c:1203-1218) into the existing tilebuf contents */
 __global__ void k_over(float4 *dst, const float4 *src, int num, int npix)
 {
@@ -318,12 +391,28 @@ int run_op(int op, int n, const float *in, size_t in_floats, float *out, size_t 
            int lutsize, float alpha)
 {
     int rc;
-    if ((rc = io_buffer(in_floats + out_floats))) return rc;
     RenderArgs a;
     if (op == OP_FIRSTVOXEL || op == OP_VOXELS)
         memset(&a, 0, sizeof a); /* the walk reads only its input */
     else
         op_args(a, lutsize, alpha);
+    if (in_floats + out_floats <= (1u << 20)) { /* per-ray sizes: through mapped host memory */
+        if ((rc = host_io(in_floats + out_floats))) return rc;
+        float *h_in = g.h_io, *h_out = g.h_io + in_floats;
+        memcpy(h_in, in, in_floats * sizeof(float));
+        if (op == OP_TF) memcpy(h_out, out, 4 * sizeof(float));
+        if (op == OP_RAYCAST)
+            hipLaunchKernelGGL(k_raycast_rays, dim3(n), dim3(64), 0, g.ctx->stream, a, n, g.h_io_dev,
+                               g.h_io_dev + in_floats);
+        else
+            hipLaunchKernelGGL(k_ops, dim3((n + 63) / 64), dim3(64), 0, g.ctx->stream, a, op, n, g.h_io_dev,
+                               g.h_io_dev + in_floats, maxs);
+        IVR_HIP(hipGetLastError());
+        IVR_HIP(hipStreamSynchronize(g.ctx->stream));
+        memcpy(out, h_out, out_floats * sizeof(float));
+        return IVR_OK;
+    }
+    if ((rc = io_buffer(in_floats + out_floats))) return rc;
     float *d_in = g.d_io, *d_out = g.d_io + in_floats;
     if (in_floats) IVR_HIP(hipMemcpyAsync(d_in, in, in_floats * sizeof(float), hipMemcpyHostToDevice, g.ctx->stream));
     if (op == OP_TF)
@@ -369,6 +458,34 @@ int lut_extent(int x, const float *s, int n)
         hi = std::max(hi, idx);
     }
     return hi + 1;
+}
+
+/* raycast calls waiting to be combined into one launch */
+struct RayCall {
+    float ray[6];
+    float out[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool done = false;
+};
+struct RayQueue {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<RayCall *> queue;
+    bool busy = false; /* a caller is casting a batch */
+} g_rays;
+
+/* one launch of OP_RAYCAST over the batch's rays */
+void cast_batch(const std::vector<RayCall *> &batch)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    const int n = (int)batch.size();
+    std::vector<float> in((size_t)n * 6), out((size_t)n * 4, 0.0f);
+    for (int k = 0; k < n; k++) memcpy(&in[(size_t)k * 6], batch[k]->ray, 6 * sizeof(float));
+    int rc = prepare(0);
+    if (!rc) rc = run_op(OP_RAYCAST, n, in.data(), in.size(), out.data(), out.size(), 0, g.hsize, mystate.alpha);
+    report(rc, "raycast");
+    for (int k = 0; k < n; k++) memcpy(batch[k]->out, &out[(size_t)k * 4], 4 * sizeof(float));
+    g.ray_batches++;
+    g.rays_cast += (unsigned long long)n;
 }
 
 ivr_color4f tf_call(int x, ivr_color4f *l, ivr_color4f c, float *s, int n, float alpha)
@@ -418,16 +535,36 @@ void inittransferfunc(int xfer)
 
 void raycast(ivr_ray3f ray, ivr_color4f *color)
 {
-    std::lock_guard<std::mutex> lk(g.mu);
-    float r[6] = {ray.orig.x, ray.orig.y, ray.orig.z, ray.dir.x, ray.dir.y, ray.dir.z};
-    float o[4] = {0, 0, 0, 0};
-    int rc = prepare(0);
-    if (!rc) rc = run_op(OP_RAYCAST, 1, r, 6, o, 4, 0, g.hsize, mystate.alpha);
-    report(rc, "raycast");
-    color->r = o[0];
-    color->g = o[1];
-    color->b = o[2];
-    color->a = o[3];
+    /* Calls from concurrent threads (the OpenMP pixel loop of createlocalimage,
+     * main.c:983-1009) are combined: the first caller to find no launch in
+     * progress takes every queued ray (its own included) and casts them in one
+     * launch; the others wait for their results.  Each ray's result is the one
+     * a launch of it alone gives. */
+    RayCall call;
+    const float r[6] = {ray.orig.x, ray.orig.y, ray.orig.z, ray.dir.x, ray.dir.y, ray.dir.z};
+    memcpy(call.ray, r, sizeof r);
+    std::unique_lock<std::mutex> lk(g_rays.mu);
+    g_rays.queue.push_back(&call);
+    while (!call.done) {
+        if (g_rays.busy) {
+            g_rays.cv.wait(lk);
+            continue;
+        }
+        g_rays.busy = true;
+        std::vector<RayCall *> batch;
+        batch.swap(g_rays.queue);
+        lk.unlock();
+        cast_batch(batch);
+        lk.lock();
+        for (RayCall *c : batch) c->done = true;
+        g_rays.busy = false;
+        g_rays.cv.notify_all();
+    }
+    lk.unlock();
+    color->r = call.out[0];
+    color->g = call.out[1];
+    color->b = call.out[2];
+    color->a = call.out[3];
 }
 
 int getsamplesonray(ivr_ray3f ray, int maxsamples, float *samples)
@@ -569,6 +706,13 @@ int ivr_compat_compositetiles(float *tilebuf, float *rcolorbuf, int num, int w, 
     IVR_HIP(hipMemcpyAsync(tilebuf, d_dst, npix * 16, hipMemcpyDeviceToHost, g.ctx->stream));
     IVR_HIP(hipStreamSynchronize(g.ctx->stream));
     return IVR_OK;
+}
+
+void ivr_compat_raycast_stats(unsigned long long *launches, unsigned long long *rays)
+{
+    std::lock_guard<std::mutex> lk(g.mu);
+    if (launches) *launches = g.ray_batches;
+    if (rays) *rays = g.rays_cast;
 }
 
 void ivr_compat_invalidate(void)

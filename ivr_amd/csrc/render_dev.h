@@ -562,17 +562,26 @@ __device__ __forceinline__ float sample_ref(const RenderArgs &a, const unsigned 
     return c0 * (1.0f - zd) + c1 * zd;
 }
 
-/* one step of transfer function `x` (transferfuncs.c:24-199) with sample
- * value s onto colour c; lut is the RGBA table (LDS or global) */
-__device__ __forceinline__ void tf_step(const RenderArgs &a, const float4 *s_lut, int x, float s,
-                                        Acc &c)
+/* the LUT entry one step of transfer function `x` (transferfuncs.c:24-168)
+ * reads for sample value s; RGBBLEND (5) reads none (-1) */
+__device__ __forceinline__ int tf_lut_index(const RenderArgs &a, int x, float s)
 {
-    if (x == 2 || x == 3) { /* SEISMIC2 / CHEM (transferfuncs.c:109-168) */
+    if (x == 5) return -1;
+    if (x == 2 || x == 3) { /* SEISMIC2 / CHEM */
         int idx = (int)(s * a.lutscale) - a.lutoffset;
         if (idx < 0) idx = 0;
         if (x == 2 && idx > a.maxlutindx) idx = a.maxlutindx;
-        idx = min(idx, a.lutsize - 1);
-        const float4 e = s_lut[idx];
+        return min(idx, a.lutsize - 1);
+    }
+    if (x == 4) return min(max((int)(s * a.lutscale), 0), a.lutsize - 1); /* BLEND: no lutoffset */
+    return min(max((int)(s * a.lutscale) - a.lutoffset, 0), a.lutsize - 1);
+}
+
+/* one step of transfer function `x` (transferfuncs.c:24-199) with sample
+ * value s and its LUT entry e (tf_lut_index) onto colour c */
+__device__ __forceinline__ void tf_apply(const RenderArgs &a, int x, float s, float4 e, Acc &c)
+{
+    if (x == 2 || x == 3) { /* SEISMIC2 / CHEM (transferfuncs.c:109-168) */
         const float al = e.w * a.alpha;
         const float om = 1.0f - al;
         c.b = om * c.b + al * e.z;
@@ -592,23 +601,17 @@ __device__ __forceinline__ void tf_step(const RenderArgs &a, const float4 *s_lut
         c.a = al + om * c.a;
         if (c.a > 1.0f) c.a = 1.0f;
     } else if (x == 1) { /* MEDXFER (transferfuncs.c:24-68) */
-        const int idx = min(max((int)(s * a.lutscale) - a.lutoffset, 0), a.lutsize - 1);
-        const float4 e = s_lut[idx];
         c.b = ((double)e.z < 0.8) ? fmaxf(e.z, c.b) : c.b;
         c.g = ((double)e.y < 0.8) ? fmaxf(e.y, c.g) : c.g;
         c.r = ((double)e.x < 0.8) ? fmaxf(e.x, c.r) : c.r;
         c.a = e.w * a.alpha;
-    } else if (x == 4) { /* BLEND (transferfuncs.c:70-84): no lutoffset */
-        const int idx = min(max((int)(s * a.lutscale), 0), a.lutsize - 1);
-        const float4 e = s_lut[idx];
+    } else if (x == 4) { /* BLEND (transferfuncs.c:70-84) */
         const float al = e.w;
         c.b = al * e.z + (1.0f - al) * c.b;
         c.g = al * e.y + (1.0f - al) * c.g;
         c.r = al * e.x + (1.0f - al) * c.r;
         c.a = al * a.alpha;
     } else { /* SEISMIC (transferfuncs.c:86-107) */
-        const int idx = min(max((int)(s * a.lutscale) - a.lutoffset, 0), a.lutsize - 1);
-        const float4 e = s_lut[idx];
         const double lb = e.z, lg = e.y, lr = e.x;
         c.b = ((lb > 0.65) && (e.z < 1.0f)) ? 0.6f : c.b;
         c.g = ((lg > 0.65) && (e.y < 1.0f)) ? 0.0f : c.g;
@@ -621,6 +624,15 @@ __device__ __forceinline__ void tf_step(const RenderArgs &a, const float4 *s_lut
         c.r = ((lr > 0.3) && (e.x < 0.5f)) ? 0.6f : c.r;
         c.a = 1.0f * a.alpha;
     }
+}
+
+/* one step of transfer function `x` with sample value s onto colour c; lut
+ * is the RGBA table (LDS or global) */
+__device__ __forceinline__ void tf_step(const RenderArgs &a, const float4 *s_lut, int x, float s, Acc &c)
+{
+    const int idx = tf_lut_index(a, x, s);
+    const float4 e = idx >= 0 ? s_lut[idx] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    tf_apply(a, x, s, e, c);
 }
 
 /* transfer functions that walk the samples from the far end (BTF) */

@@ -86,11 +86,12 @@ def test_frames_through_dropin(refgpu):
     z = np.load(os.path.join(G, "compat_images.npz"))
     luts = {"tf5": z["lut_tf5"], "medical": z["lut_medical"], "jet": z["lut_jet"]}
     W, H = META["width"], META["height"]
+    stats = lambda: [(refgpu.ivr_compat_raycast_stats(C.byref(a), C.byref(b)), a.value, b.value)[1:]  # noqa: E731
+                     for a, b in [(C.c_ulonglong(), C.c_ulonglong())]][0]
     for use_gpu_frame in (1, 0):
         assert refgpu.ivrref_frame_on_gpu(use_gpu_frame) == use_gpu_frame
         cases = list(enumerate(META["compat_cases"]))
-        if not use_gpu_frame:
-            cases = cases[:2]  # the per-pixel path launches once per ray
+        l0, r0 = stats()
         for k, case in cases:
             print("drop-in frame", use_gpu_frame, k, case, flush=True)
             st = ivr.state_from_angles(case["az"], case["el"], W, H)
@@ -104,6 +105,12 @@ def test_frames_through_dropin(refgpu):
                                       C.c_float(case["alpha"]), W, H, f(out), rect)
             assert rc == 0
             assert np.array_equal(out.view(np.uint32), z["images"][k].view(np.uint32)), (use_gpu_frame, case)
+        l1, r1 = stats()
+        if not use_gpu_frame:
+            # the reference's pixel loop called raycast from its OpenMP threads; concurrent calls
+            # shared launches
+            print("per-pixel raycast: %d rays in %d launches" % (r1 - r0, l1 - l0), flush=True)
+            assert r1 - r0 > 0 and (os.cpu_count() == 1 or l1 - l0 < r1 - r0)
     refgpu.ivrref_frame_on_gpu(0)
 
 
