@@ -57,29 +57,51 @@ struct Enc {
     unsigned n1[SG_L];  /* N+1 for dense levels, 0 for hashed */
 };
 
-__device__ __forceinline__ unsigned corner_index(const Enc &e, int l, int x, int y, int z)
+/* the level a lane group works on: l = a + step * g for lane group g, with
+ * its resolution and dense edge picked from the four candidates by g, so
+ * the level tables (kernel arguments) are only ever indexed uniformly */
+struct Lvl {
+    int l, res;
+    unsigned n1;
+};
+
+__device__ __forceinline__ int sel4(int g, int v0, int v1, int v2, int v3)
 {
-    const unsigned n1 = e.n1[l];
+    return g == 0 ? v0 : (g == 1 ? v1 : (g == 2 ? v2 : v3));
+}
+
+__device__ __forceinline__ Lvl lvl_of(const Enc &e, int g, int a, int step)
+{
+    Lvl L;
+    L.l = a + step * g;
+    L.res = sel4(g, e.res[a], e.res[a + step], e.res[a + 2 * step], e.res[a + 3 * step]);
+    L.n1 = (unsigned)sel4(g, (int)e.n1[a], (int)e.n1[a + step], (int)e.n1[a + 2 * step], (int)e.n1[a + 3 * step]);
+    return L;
+}
+
+__device__ __forceinline__ unsigned corner_index(unsigned T, const Lvl &L, int x, int y, int z)
+{
+    const unsigned n1 = L.n1;
     if (n1) return (unsigned)x + n1 * ((unsigned)y + n1 * (unsigned)z);
     /* hashed levels: the 4x4x4 block of corners hashes to a slot of 64
      * entries (the spatial hash of Mueller et al. 2022 on block coordinates)
      * holding its corners in Morton order, so a cell's 8 corners take about
      * 2.3 cache lines instead of about 4 */
     const unsigned bx = (unsigned)x >> 2, by = (unsigned)y >> 2, bz = (unsigned)z >> 2;
-    const unsigned blk = (bx * 1u ^ by * 2654435761u ^ bz * 805459861u) & ((e.T >> 6) - 1u);
+    const unsigned blk = (bx * 1u ^ by * 2654435761u ^ bz * 805459861u) & ((T >> 6) - 1u);
     const unsigned local = (x & 1) | (y & 1) << 1 | (z & 1) << 2 | (x & 2) << 2 | (y & 2) << 3 | (z & 2) << 4;
     return blk << 6 | local;
 }
 
-/* corner 0 and fractional weights of level l at local position x in [0,1] */
-__device__ __forceinline__ void level_cell(const Enc &e, int l, const float *x, int *v0, float *w)
+/* corner 0 and fractional weights of level L at local position x in [0,1] */
+__device__ __forceinline__ void level_cell(const Lvl &L, const float *x, int *v0, float *w)
 {
-    const float N = (float)e.res[l];
+    const float N = (float)L.res;
 #pragma unroll
     for (int a = 0; a < 3; a++) {
         const float u = x[a] * N;
         int v = (int)floorf(u);
-        v = min(max(v, 0), e.res[l] - 1);
+        v = min(max(v, 0), L.res - 1);
         v0[a] = v;
         w[a] = u - (float)v;
     }
@@ -88,41 +110,41 @@ __device__ __forceinline__ void level_cell(const Enc &e, int l, const float *x, 
 __device__ __forceinline__ float2 table_at(const float2 *tb, unsigned i) { return tb[i]; }
 __device__ __forceinline__ float2 table_at(const __half2 *tb, unsigned i) { return __half22float2(tb[i]); }
 
-/* level l of the encoding at local position x: trilinear blend of the 8
+/* level L of the encoding at local position x: trilinear blend of the 8
  * corner features, from the fp32 tables (TT = float2, training) or their
  * half-precision copy (TT = __half2, rendering: half the bytes per corner) */
 template <typename TT = float2>
-__device__ __forceinline__ float2 encode_level(const Enc &e, int brick, int l, const float *x)
+__device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lvl &L, const float *x)
 {
     int v0[3];
     float w[3];
-    level_cell(e, l, x, v0, w);
-    const size_t base = ((size_t)brick * SG_L + l) * (size_t)e.T;
+    level_cell(L, x, v0, w);
+    const size_t base = ((size_t)brick * SG_L + L.l) * (size_t)e.T;
     const TT *tb = sizeof(TT) == sizeof(float2) ? (const TT *)e.table + base : (const TT *)e.table16 + base;
     float2 f = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
         const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const float2 t = table_at(tb, corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz));
+        const float2 t = table_at(tb, corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz));
         f.x = fmaf(wc, t.x, f.x);
         f.y = fmaf(wc, t.y, f.y);
     }
     return f;
 }
 
-__device__ __forceinline__ void scatter_level(const Enc &e, float *gtable, int brick, int l, const float *x,
+__device__ __forceinline__ void scatter_level(const Enc &e, float *gtable, int brick, const Lvl &L, const float *x,
                                               float g0, float g1)
 {
     int v0[3];
     float w[3];
-    level_cell(e, l, x, v0, w);
-    float *tb = gtable + ((size_t)brick * SG_L + l) * (size_t)e.T * SG_F;
+    level_cell(L, x, v0, w);
+    float *tb = gtable + ((size_t)brick * SG_L + L.l) * (size_t)e.T * SG_F;
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
         const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const unsigned idx = corner_index(e, l, v0[0] + dx, v0[1] + dy, v0[2] + dz);
+        const unsigned idx = corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz);
         unsafeAtomicAdd(tb + (size_t)idx * 2, wc * g0);
         unsafeAtomicAdd(tb + (size_t)idx * 2 + 1, wc * g1);
     }
@@ -155,7 +177,7 @@ __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, b
     for (int nt = 0; nt < 4; nt++) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float2 f = encode_level<TT>(e, s.b[nt], 4 * g + q, s.x[nt]);
+            const float2 f = encode_level<TT>(e, s.b[nt], lvl_of(e, g, q, 4), s.x[nt]);
             E[nt][2 * q] = (__bf16)f.x;
             E[nt][2 * q + 1] = (__bf16)f.y;
         }
@@ -227,6 +249,54 @@ __device__ __forceinline__ void mlp_forward(const Mlp &m, const bf16x8 E[4], Fwd
     }
 }
 
+/* The forward pass of inference: y of mlp_forward with the same float
+ * operations in the same order (layer 3's sums run over mt, then i, for
+ * each nt), but each layer-2 output tile is folded into y as soon as it is
+ * made instead of being kept for a backward pass, which leaves 64 fewer
+ * registers live across the product. */
+__device__ __forceinline__ void mlp_infer(const Mlp &m, const bf16x8 E[4], float y[4])
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+    bf16x8 H[2][4];
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a = m.frag[(SG_FRAG_F1 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = mfma(a, E[nt], zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = c[i] + m.p[SG_P_B1 + mt * 16 + g * 4 + i];
+                H[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
+            }
+        }
+    }
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a0 = m.frag[(SG_FRAG_F2 + mt * 2 + 0) * 64 + lane];
+        const bf16x8 a1 = m.frag[(SG_FRAG_F2 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = mfma(a0, H[0][nt], zero);
+            c = mfma(a1, H[1][nt], c);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = c[i] + m.p[SG_P_B2 + mt * 16 + g * 4 + i];
+                acc[nt] = fmaf(m.p[SG_P_W3 + mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc[nt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        float v = acc[nt];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        y[nt] = v + m.p[SG_P_B3];
+    }
+}
+
 /* own sample's value from the y[nt] a lane holds (its sample is nt = l/16) */
 __device__ __forceinline__ float own(const float v[4])
 {
@@ -244,9 +314,9 @@ __device__ __forceinline__ float sg_infer(const Enc &enc, const Mlp &mlp, const 
     gather_samples(x, b, ss);
     bf16x8 E[4];
     encode_frags<__half2>(enc, ss, E);
-    Fwd f;
-    mlp_forward(mlp, E, f);
-    return own(f.y);
+    float y[4];
+    mlp_infer(mlp, E, y);
+    return own(y);
 }
 
 /* ---------------------------------------------------------------- training */
@@ -410,8 +480,8 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
                 const bool vn = (nt * 16 + (lane & 15)) + ((grp * 2 + wave) * 64) < A.nsamples;
                 if (vn) {
                     /* inputs mt*16 + 4g + i are levels mt*8 + 2g + {0,1} */
-                    scatter_level(A.enc, A.gtable, ss.b[nt], mt * 8 + 2 * g, ss.x[nt], c[0], c[1]);
-                    scatter_level(A.enc, A.gtable, ss.b[nt], mt * 8 + 2 * g + 1, ss.x[nt], c[2], c[3]);
+                    scatter_level(A.enc, A.gtable, ss.b[nt], lvl_of(A.enc, g, mt * 8, 2), ss.x[nt], c[0], c[1]);
+                    scatter_level(A.enc, A.gtable, ss.b[nt], lvl_of(A.enc, g, mt * 8 + 1, 2), ss.x[nt], c[2], c[3]);
                 }
             }
         }
