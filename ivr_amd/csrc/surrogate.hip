@@ -24,6 +24,13 @@
  * Encoding work is spread the same way: lane l computes levels 4*(l/16) ..
  * 4*(l/16)+3 of the samples l%16, 16+l%16, 32+l%16, 48+l%16, which is the
  * B-operand fragment of the first layer.
+ *
+ * The hash-table gradient (two fp32 atomics per corner, level and sample)
+ * is scattered by 16 lanes per (sample, level): each lane adds one feature
+ * of one of the cell's 8 corners, so the 64 atomics of an instruction fall
+ * on the few cache lines of four cells (a 4x4x4 block keeps a cell's
+ * corners within about 2.3 lines) rather than on 64 unrelated lines; the
+ * training step went from 219 to 77 ms for 2^24 samples on 8 bricks.
  */
 #include <cstring>
 #include <cmath>
@@ -131,23 +138,6 @@ __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lv
         f.y = fmaf(wc, t.y, f.y);
     }
     return f;
-}
-
-__device__ __forceinline__ void scatter_level(const Enc &e, float *gtable, int brick, const Lvl &L, const float *x,
-                                              float g0, float g1)
-{
-    int v0[3];
-    float w[3];
-    level_cell(L, x, v0, w);
-    float *tb = gtable + ((size_t)brick * SG_L + L.l) * (size_t)e.T * SG_F;
-#pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-        const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const unsigned idx = corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz);
-        unsafeAtomicAdd(tb + (size_t)idx * 2, wc * g0);
-        unsafeAtomicAdd(tb + (size_t)idx * 2 + 1, wc * g1);
-    }
 }
 
 /* first-layer B fragments: lane l, sample nt*16+l%16, features 8g..8g+7 */
@@ -338,6 +328,7 @@ struct TrainArgs {
 };
 
 #define SG_LDS_STRIDE 72 /* bf16 per staged row: 64 samples + pad */
+#define SG_GSTRIDE 33    /* floats per sample row of the staged encoding gradient */
 
 /* Jarzynski & Olano, "Hash Functions for GPU Rendering" (JCGT 2020): pcg hash */
 __device__ __forceinline__ unsigned pcg_hash(unsigned v)
@@ -468,23 +459,6 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
                 }
             }
         }
-        /* dET = W1^T dZ1T, scattered into the hash tables */
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 0) * 64 + lane];
-            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 1) * 64 + lane];
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) {
-                f32x4 c = mfma(a0, D1[0][nt], zero);
-                c = mfma(a1, D1[1][nt], c);
-                const bool vn = (nt * 16 + (lane & 15)) + ((grp * 2 + wave) * 64) < A.nsamples;
-                if (vn) {
-                    /* inputs mt*16 + 4g + i are levels mt*8 + 2g + {0,1} */
-                    scatter_level(A.enc, A.gtable, ss.b[nt], lvl_of(A.enc, g, mt * 8, 2), ss.x[nt], c[0], c[1]);
-                    scatter_level(A.enc, A.gtable, ss.b[nt], lvl_of(A.enc, g, mt * 8 + 1, 2), ss.x[nt], c[2], c[3]);
-                }
-            }
-        }
         /* encodings as ET[feature][sample] for the first layer's weight gradient */
 #pragma unroll
         for (int nt = 0; nt < 4; nt++)
@@ -509,6 +483,58 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
                     const bf16x8 be = *(const bf16x8 *)&sE[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
                     gW1[mo][ni] = mfma(a1, be, gW1[mo][ni]);
                 }
+            }
+        }
+        __syncthreads();
+        /* dET = W1^T dZ1T into LDS (the weight-gradient operands are done with),
+         * with each sample's position and brick (-1: past the batch) */
+        float *sG = (float *)sDZ2;  /* [64 samples][SG_GSTRIDE] */
+        float *sX = (float *)sH1;   /* [64 samples][4]: x, y, z, brick */
+#pragma unroll
+        for (int mt = 0; mt < 2; mt++) {
+            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                f32x4 c = mfma(a0, D1[0][nt], zero);
+                c = mfma(a1, D1[1][nt], c);
+#pragma unroll
+                for (int i = 0; i < 4; i++) sG[(nt * 16 + (lane & 15)) * SG_GSTRIDE + mt * 16 + g * 4 + i] = c[i];
+            }
+        }
+        if (g == 0) {
+#pragma unroll
+            for (int nt = 0; nt < 4; nt++) {
+                const int s = nt * 16 + lane;
+                const bool vn = s + ((grp * 2 + wave) * 64) < A.nsamples;
+                sX[s * 4 + 0] = ss.x[nt][0];
+                sX[s * 4 + 1] = ss.x[nt][1];
+                sX[s * 4 + 2] = ss.x[nt][2];
+                sX[s * 4 + 3] = __int_as_float(vn ? ss.b[nt] : -1);
+            }
+        }
+        __syncthreads();
+        /* the table gradient: 16 lanes per (sample, level), one lane per corner and
+         * feature, so the atomics of one instruction fall on the few cache lines
+         * of four cells instead of on 64 unrelated lines */
+        {
+            const int corner = lane & 7, feat = (lane >> 3) & 1;
+            const int dx = corner & 1, dy = (corner >> 1) & 1, dz = corner >> 2;
+#pragma unroll 4
+            for (int round = 0; round < 256; round++) {
+                const int s = round >> 2, a = (round & 3) * 4;
+                const int bb = __float_as_int(sX[s * 4 + 3]);
+                if (bb < 0) continue; /* uniform over the wavefront: one sample per round */
+                const float x[3] = {sX[s * 4 + 0], sX[s * 4 + 1], sX[s * 4 + 2]};
+                const Lvl L = lvl_of(A.enc, g, a, 1);
+                int v0[3];
+                float w[3];
+                level_cell(L, x, v0, w);
+                const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
+                const unsigned idx = corner_index(A.enc.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz);
+                const float gr = sG[s * SG_GSTRIDE + L.l * 2 + feat];
+                float *tb = A.gtable + ((size_t)bb * SG_L + L.l) * (size_t)A.enc.T * SG_F;
+                unsafeAtomicAdd(tb + (size_t)idx * 2 + feat, wc * gr);
             }
         }
         __syncthreads();
