@@ -4,11 +4,11 @@
  * The reference render server handles one STATE at a time: receive, render
  * all pixels, composite, send (main.c:206-250).  Here requests of different
  * sizes, cameras, transfer functions and resident volumes share persistent
- * launches: k_render_batch's workgroups pull 16x16-pixel tiles from a
- * device counter over the concatenated tile lists of every request in the
- * batch, stage that request's arguments, LUT and brick table in LDS and
- * render the tile with the same config-mode march as the single-frame
- * kernel, so each image is bit-identical alone and in a mix.
+ * launches: k_render_batch's workgroups pull pixel blocks from XCD-affine
+ * queues over the concatenated block lists of every request in the batch,
+ * stage that request's arguments, LUT and brick table in LDS and render the
+ * block with the same config-mode march as the single-frame kernel, so each
+ * image is bit-identical alone and in a mix.
  *
  * Host side: requests wait in a FIFO, are admitted into a slot while one is
  * free and the HBM budget allows, and the admitted-but-unlaunched slots
@@ -31,57 +31,78 @@ using namespace ivr_dev;
 
 #define BATCH_LUT 256
 
+/* The batch's pixel blocks (16 x 16 for fp32 volumes, 64 x 4 for fp16, with
+ * the wavefront shapes of the single-frame kernel) are numbered over the
+ * concatenated requests and dealt to 8 queues in runs of 8 consecutive
+ * blocks, queue q holding runs q, q + 8, ...; a workgroup takes blocks from
+ * the queue of its XCD (workgroups are dealt to the XCDs round robin) and,
+ * when that is empty, from the others in turn, so each L2 sees runs of
+ * neighbouring blocks while every XCD keeps busy.  A block's request
+ * arguments, LUT and brick table are staged in LDS when the request
+ * changes.  Every workgroup leaves once all eight queues are exhausted. */
+__device__ __forceinline__ long long queue_block(int q, unsigned k)
+{
+    return ((long long)(k >> 3) * 8 + q) * 8 + (k & 7);
+}
+
 __global__ __launch_bounds__(IVR_BLOCK) void k_render_batch(const RenderArgs *args, const int *prefix, int nslots,
-                                                            int total, unsigned *counter)
+                                                            int total, unsigned *queues)
 {
     __shared__ float4 s_lut[BATCH_LUT];
     __shared__ unsigned long long s_boff[IVR_MAX_BRICKS / 8];
     __shared__ RenderArgs s_a;
     __shared__ int s_tile, s_slot;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            const int t = (int)atomicAdd(counter, 1u);
-            int sl = 0;
-            while (sl + 1 < nslots && prefix[sl + 1] <= t) sl++;
-            s_tile = t;
-            s_slot = sl;
-        }
-        __syncthreads();
-        const int tile = s_tile, slot = s_slot;
-        if (tile >= total) break;
-        {
-            const unsigned *src = (const unsigned *)&args[slot];
-            unsigned *dst = (unsigned *)&s_a;
-            for (int w = threadIdx.x; w < (int)(sizeof(RenderArgs) / 4); w += blockDim.x) dst[w] = src[w];
-        }
-        __syncthreads();
-        const RenderArgs &a = s_a;
-        for (int t = threadIdx.x; t < a.lutsize; t += blockDim.x) s_lut[t] = a.lut[t];
-        for (int t = threadIdx.x; t < a.nbricks; t += blockDim.x) s_boff[t] = a.boff[t];
-        __syncthreads();
-        const int lt = tile - prefix[slot];
-        const int bx = lt % a.nblk_x, by = lt / a.nblk_x;
-        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-        const int j = bx * 16 + (w & 1) * 8 + (lane & 7);
-        const int i = by * 16 + (w >> 1) * 8 + (lane >> 3);
-        Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        unsigned cnt = 0, fetched = 0;
-        if (i < a.H && j < a.W) {
-            float d[3];
-            cfg_dir(a, i, j, d);
-            const bool skip = a.cellempty != nullptr;
-            if (a.dtype_f16) {
-                if (skip) march_config<_Float16, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
-                else march_config<_Float16, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
-            } else {
-                if (skip) march_config<float, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
-                else march_config<float, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+    int staged = -1;
+    const int xcd = (int)(blockIdx.x & 7);
+    for (int qi = 0; qi < 8; qi++) {
+        const int q = (xcd + qi) & 7;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                const long long t = queue_block(q, atomicAdd(&queues[q], 1u));
+                int sl = 0;
+                if (t < total)
+                    while (sl + 1 < nslots && prefix[sl + 1] <= (int)t) sl++;
+                s_tile = t < total ? (int)t : -1;
+                s_slot = sl;
             }
-            a.out8[(size_t)i * a.W + j] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
+            __syncthreads();
+            const int tile = s_tile, slot = s_slot;
+            if (tile < 0) {
+                __syncthreads(); /* every thread has read s_tile before it is written again */
+                break;
+            }
+            if (slot != staged) {
+                const unsigned *src = (const unsigned *)&args[slot];
+                unsigned *dst = (unsigned *)&s_a;
+                for (int w = threadIdx.x; w < (int)(sizeof(RenderArgs) / 4); w += blockDim.x) dst[w] = src[w];
+                __syncthreads();
+                for (int t = threadIdx.x; t < s_a.lutsize; t += blockDim.x) s_lut[t] = s_a.lut[t];
+                for (int t = threadIdx.x; t < s_a.nbricks; t += blockDim.x) s_boff[t] = s_a.boff[t];
+                __syncthreads();
+                staged = slot;
+            }
+            const RenderArgs &a = s_a;
+            int i, j;
+            block_pixel(a, tile - prefix[slot], i, j); /* a.xcd_chunk = 0: blocks in order */
+            Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
+            unsigned cnt = 0, fetched = 0;
+            if (i < a.H && j < a.W) {
+                float d[3];
+                cfg_dir(a, i, j, d);
+                const bool skip = a.cellempty != nullptr;
+                if (a.dtype_f16) {
+                    if (skip) march_config<_Float16, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                    else march_config<_Float16, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                } else {
+                    if (skip) march_config<float, true>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                    else march_config<float, false>(a, s_lut, s_boff, a.o, d, acc, cnt, fetched);
+                }
+                a.out8[(size_t)i * a.W + j] = make_uchar4(quant8(acc.r), quant8(acc.g), quant8(acc.b), quant8(acc.a));
+            }
+            count_add(a.counters, 0, cnt);
+            count_add(a.counters, 1, fetched);
+            __syncthreads();
         }
-        count_add(a.counters, 0, cnt);
-        count_add(a.counters, 1, fetched);
-        __syncthreads();
     }
 }
 
@@ -123,7 +144,7 @@ struct Batch {
     std::vector<int> slots;
     RenderArgs *d_args = nullptr, *h_args = nullptr;
     int *d_prefix = nullptr, *h_prefix = nullptr;
-    unsigned *d_counter = nullptr;
+    unsigned *d_counter = nullptr; /* the 8 block queues of k_render_batch */
     /* surrogate requests of the batch: their frame items, rendered by one launch */
     unsigned char *d_items = nullptr, *h_items = nullptr;
     int *d_sgprefix = nullptr, *h_sgprefix = nullptr;
@@ -198,7 +219,7 @@ int batcher_init(ivr_batcher *b)
         IVR_HIP(hipEventCreateWithFlags(&bt.done, hipEventDisableTiming));
         if ((rc = ivr_dalloc(ctx, (void **)&bt.d_args, sizeof(RenderArgs) * max_slots)) ||
             (rc = ivr_dalloc(ctx, (void **)&bt.d_prefix, sizeof(int) * (max_slots + 1))) ||
-            (rc = ivr_dalloc(ctx, (void **)&bt.d_counter, sizeof(unsigned))))
+            (rc = ivr_dalloc(ctx, (void **)&bt.d_counter, 8 * sizeof(unsigned))))
             return rc;
         IVR_HIP(hipHostMalloc((void **)&bt.h_args, sizeof(RenderArgs) * max_slots, hipHostMallocDefault));
         IVR_HIP(hipHostMalloc((void **)&bt.h_prefix, sizeof(int) * (max_slots + 1), hipHostMallocDefault));
@@ -233,7 +254,7 @@ int ivr_batcher_destroy(ivr_batcher *b)
         if (bt.done) (void)hipEventDestroy(bt.done);
         ivr_dfree(b->ctx, bt.d_args, sizeof(RenderArgs) * b->max_slots);
         ivr_dfree(b->ctx, bt.d_prefix, sizeof(int) * (b->max_slots + 1));
-        ivr_dfree(b->ctx, bt.d_counter, sizeof(unsigned));
+        ivr_dfree(b->ctx, bt.d_counter, 8 * sizeof(unsigned));
         if (bt.h_args) (void)hipHostFree(bt.h_args);
         if (bt.h_prefix) (void)hipHostFree(bt.h_prefix);
         ivr_dfree(b->ctx, bt.d_items, ivr_surrogate_item_bytes() * b->max_slots);
@@ -474,7 +495,8 @@ int launch_slots(ivr_batcher *b, Batch *bt, const std::vector<int> &ready)
         ivr_cam_setup(&d.state, v->n[0], v->n[1], v->n[2], &cam);
         RenderArgs &a = bt->h_args[q2];
         ivr_fill_args(a, cam, v, &tf, &d, &fr);
-        a.nblk_x = (fr.W + 15) / 16;
+        ivr_set_grid(a, b->ctx, fr.W, fr.H); /* the single-frame kernel's block and wavefront shapes */
+        a.xcd_chunk = 0;                     /* blocks come from the queues in order */
         a.cellempty = d.skip ? s.d_cells : nullptr;
         if (d.skip) {
             int rc = ivr_launch_cells_on(st, v, a, s.d_cells);
@@ -482,7 +504,7 @@ int launch_slots(ivr_batcher *b, Batch *bt, const std::vector<int> &ready)
         }
         IVR_HIP(hipMemsetAsync(s.d_ctr, 0, IVR_CTR_BYTES, st));
         bt->h_prefix[q2] = total;
-        total += a.nblk_x * ((fr.H + 15) / 16);
+        total += a.nblk;
         s.launched = true;
     }
     if (nsg > 0) {
@@ -496,7 +518,7 @@ int launch_slots(ivr_batcher *b, Batch *bt, const std::vector<int> &ready)
         bt->h_prefix[nplain] = total;
         IVR_HIP(hipMemcpyAsync(bt->d_args, bt->h_args, sizeof(RenderArgs) * nplain, hipMemcpyHostToDevice, st));
         IVR_HIP(hipMemcpyAsync(bt->d_prefix, bt->h_prefix, sizeof(int) * (nplain + 1), hipMemcpyHostToDevice, st));
-        IVR_HIP(hipMemsetAsync(bt->d_counter, 0, sizeof(unsigned), st));
+        IVR_HIP(hipMemsetAsync(bt->d_counter, 0, 8 * sizeof(unsigned), st));
         const int grid = std::min(total, 2048);
         hipLaunchKernelGGL(k_render_batch, dim3(grid), dim3(IVR_BLOCK), 0, st, bt->d_args, bt->d_prefix, nplain,
                            total, bt->d_counter);
