@@ -216,6 +216,10 @@ typedef struct ivr_surrogate_params {
     float beta1, beta2, eps;
     float init_scale; /* tables start U[-init_scale, init_scale]                */
     uint64_t seed;
+    /* a surrogate brick (one hash grid) spans 2^log2_group volume bricks per
+     * axis, in [0, 4]: 0 gives one grid per volume brick, larger values a
+     * representation 8^log2_group times smaller over the same volume */
+    int log2_group;
 } ivr_surrogate_params;
 
 void ivr_surrogate_defaults(ivr_surrogate_params *sp);
@@ -256,6 +260,12 @@ int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const
 /* versioned binary checkpoint of parameters and Adam state */
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
+/* keep only what rendering reads: the half-precision tables, the MLP and the
+ * macrocell ranges (2 bytes per table parameter); the fp32 tables, their
+ * gradients and Adam moments (16 bytes per parameter) are released.  Render,
+ * evaluate, eval_points, set_fallback and info work as before; train, grad,
+ * adam, get, set, save and load then fail with IVR_ERR_UNSUPPORTED. */
+int ivr_surrogate_freeze(ivr_ctx *ctx, ivr_surrogate *s);
 
 /* ---- steering-request batcher: continuous batching of render requests of
  * any size, camera, transfer function and resident volume into persistent

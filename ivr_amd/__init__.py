@@ -56,7 +56,7 @@ class FrameStats(C.Structure):
 class SurrogateParams(C.Structure):
     _fields_ = [("log2_table", C.c_int), ("base_res", C.c_int), ("growth", C.c_float), ("lr", C.c_float),
                 ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
-                ("init_scale", C.c_float), ("seed", C.c_uint64)]
+                ("init_scale", C.c_float), ("seed", C.c_uint64), ("log2_group", C.c_int)]
 
 
 SG_NPARAM = 64 * 32 + 64 + 64 * 64 + 64 + 64 + 1
@@ -140,6 +140,7 @@ def _load():
         "ivr_surrogate_eval_points": (C.c_int, [P, P, C.c_int, P, P, P]),
         "ivr_surrogate_save": (C.c_int, [P, P, C.c_char_p]),
         "ivr_surrogate_load": (C.c_int, [P, P, C.c_char_p]),
+        "ivr_surrogate_freeze": (C.c_int, [P, P]),
         "ivr_batcher_create": (C.c_int, [P, C.c_ulonglong, C.c_int, C.c_int, C.c_int, C.POINTER(P)]),
         "ivr_batcher_destroy": (C.c_int, [P]),
         "ivr_batcher_add_volume": (C.c_int, [P, P]),
@@ -502,6 +503,11 @@ class Surrogate:
 
     def load(self, path):
         check(lib.ivr_surrogate_load(self.ctx.h, self.h, path.encode()))
+
+    def freeze(self):
+        """release the training state: only the half-precision tables, the MLP
+        and the macrocell ranges stay resident"""
+        check(lib.ivr_surrogate_freeze(self.ctx.h, self.h))
 
     def close(self):
         if self.h:

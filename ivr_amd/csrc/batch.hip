@@ -289,15 +289,16 @@ int ivr_batcher_add_volume(ivr_batcher *b, const ivr_volume *vol)
 int ivr_batcher_add_surrogate(ivr_batcher *b, const ivr_surrogate *sg, const ivr_volume *vol)
 {
     if (!b || !sg) return ivr_fail(IVR_ERR_ARG, "null argument");
-    if (vol && (vol->nbricks != sg->nbricks || vol->B != sg->B))
-        return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    if (vol && !ivr_surrogate_matches(sg, vol)) return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
     std::vector<unsigned char> h(sg->nbricks);
     IVR_HIP(hipMemcpy(h.data(), sg->d_fallback, sg->nbricks, hipMemcpyDeviceToHost));
     for (unsigned char c : h)
         if (c && !vol) return ivr_fail(IVR_ERR_ARG, "a brick falls back to the plain volume, which was not given");
-    /* the network's resident bytes: hash tables (fp32 and the half copy it renders from),
-     * parameters, operand fragments, cell ranges */
-    const unsigned long long bytes = sg->ntable * 6ull + SG_NPARAM * 4ull + SG_NFRAG * 1024ull +
+    /* the network's resident bytes: the half-precision tables it renders from (plus, unless
+     * frozen, the fp32 tables, their gradients and Adam moments), parameters, operand
+     * fragments, cell ranges */
+    const unsigned long long bytes = sg->ntable * (sg->frozen ? 2ull : 18ull) +
+                                     SG_NPARAM * (sg->frozen ? 4ull : 16ull) + SG_NFRAG * 1024ull +
                                      sg->ncells * 9ull + sg->nbricks;
     if (b->used + bytes > b->budget)
         return ivr_fail(IVR_ERR_BUDGET, "surrogate of %llu bytes exceeds the HBM budget", bytes);

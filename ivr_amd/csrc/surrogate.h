@@ -31,8 +31,11 @@
 struct ivr_surrogate {
     ivr_surrogate_params sp;
     unsigned T = 0;
-    int nbricks = 0, B = 0;
+    /* surrogate bricks: edge B = 2^bshift voxels (the volume's brick edge << sp.log2_group),
+     * nb per axis, brick (bx,by,bz) owning floor indices [b*B, b*B+B) like a volume brick */
+    int nbricks = 0, B = 0, bshift = 0;
     int n[3] = {0, 0, 0}, nb[3] = {0, 0, 0};
+    bool frozen = false; /* ivr_surrogate_freeze: fp32 tables and optimiser state released */
     int res[SG_L] = {};
     unsigned dense_n1[SG_L] = {};
     size_t ntable = 0;
@@ -51,6 +54,9 @@ struct ivr_surrogate {
     float2 *d_cellrange = nullptr;
     unsigned char *d_cellempty = nullptr;
 };
+
+/* is v the volume (dims and brick edge) that s was made for */
+bool ivr_surrogate_matches(const ivr_surrogate *s, const ivr_volume *v);
 
 /* a surrogate frame on stream st with the caller's macrocell buffer (ncells
  * bytes, used with skipping) and per-brick fallback flags (device, nbricks

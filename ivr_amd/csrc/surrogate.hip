@@ -32,6 +32,7 @@
  * corners within about 2.3 lines) rather than on 64 unrelated lines; the
  * training step went from 219 to 77 ms for 2^24 samples on 8 bricks.
  */
+#include <cstddef>
 #include <cstring>
 #include <cmath>
 #include <vector>
@@ -313,6 +314,8 @@ __device__ __forceinline__ float sg_infer(const Enc &enc, const Mlp &mlp, const 
 
 struct TrainArgs {
     RenderArgs va;        /* plain volume (brick table, dims) for targets */
+    int snb[3], sbshift;  /* surrogate bricks: per axis, log2 of the edge */
+    int snbricks;
     Enc enc;
     Mlp mlp;
     float *gtable;
@@ -387,17 +390,17 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
             t = A.ts[s];
         } else {
             const RenderArgs &va = A.va;
-            b = (int)(sid % va.nbricks);
-            const int bc[3] = {b % va.nb[0], (b / va.nb[0]) % va.nb[1], b / (va.nb[0] * va.nb[1])};
+            b = (int)(sid % A.snbricks);
+            const int bc[3] = {b % A.snb[0], (b / A.snb[0]) % A.snb[1], b / (A.snb[0] * A.snb[1])};
             float gv[3];
 #pragma unroll
             for (int r = 0; r < 3; r++) {
                 const unsigned h = pcg_hash(A.seed ^ pcg_hash(A.step * 3u + (unsigned)r) ^ pcg_hash((unsigned)sid * 3u + r));
                 const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
-                const int lo = bc[r] << va.bshift;
-                const float ext = (float)min(1 << va.bshift, (va.n[r] - 1) - lo);
+                const int lo = bc[r] << A.sbshift;
+                const float ext = (float)min(1 << A.sbshift, (va.n[r] - 1) - lo);
                 gv[r] = (float)lo + u * ext;
-                x[r] = u * ext / (float)(1 << va.bshift);
+                x[r] = u * ext / (float)(1 << A.sbshift);
             }
             t = field_at(va, va.boff, gv[0], gv[1], gv[2]);
         }
@@ -684,19 +687,28 @@ int refresh_frags(ivr_ctx *ctx, ivr_surrogate *s)
 
 } // namespace
 
+bool ivr_surrogate_matches(const ivr_surrogate *s, const ivr_volume *v)
+{
+    return v->n[0] == s->n[0] && v->n[1] == s->n[1] && v->n[2] == s->n[2] &&
+           v->bshift + s->sp.log2_group == s->bshift;
+}
+
 int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogate_params *sp, ivr_surrogate **out)
 {
     if (!ctx || !vol || !sp || !out) return ivr_fail(IVR_ERR_ARG, "null argument");
     if (sp->log2_table < 10 || sp->log2_table > 24) return ivr_fail(IVR_ERR_ARG, "log2_table %d", sp->log2_table);
+    if (sp->log2_group < 0 || sp->log2_group > 4) return ivr_fail(IVR_ERR_ARG, "log2_group %d", sp->log2_group);
     ivr_surrogate *s = new ivr_surrogate();
     s->sp = *sp;
     s->T = 1u << sp->log2_table;
-    s->nbricks = vol->nbricks;
+    s->bshift = vol->bshift + sp->log2_group;
+    s->B = 1 << s->bshift;
+    s->nbricks = 1;
     for (int r = 0; r < 3; r++) {
-        s->nb[r] = vol->nb[r];
         s->n[r] = vol->n[r];
+        s->nb[r] = ((vol->n[r] - 2) >> s->bshift) + 1; /* bricks owning floor indices 0 .. n-2 */
+        s->nbricks *= s->nb[r];
     }
-    s->B = vol->B;
     for (int l = 0; l < SG_L; l++) {
         s->res[l] = (int)std::floor(sp->base_res * std::pow((double)sp->growth, l));
         const unsigned long long n1 = (unsigned long long)s->res[l] + 1;
@@ -782,6 +794,9 @@ int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long lon
     TrainArgs A;
     memset(&A, 0, sizeof A);
     if (vol) volume_args(A.va, vol);
+    for (int r = 0; r < 3; r++) A.snb[r] = s->nb[r];
+    A.sbshift = s->bshift;
+    A.snbricks = s->nbricks;
     A.enc = make_enc(s);
     A.mlp.frag = (const bf16x8 *)s->frag;
     A.mlp.p = s->params;
@@ -826,7 +841,8 @@ int ivr_surrogate_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, i
                         float *loss_out)
 {
     if (!ctx || !s || !vol) return ivr_fail(IVR_ERR_ARG, "null argument");
-    if (vol->nbricks != s->nbricks || vol->B != s->B) return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
+    if (!ivr_surrogate_matches(s, vol)) return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
     if (samples < 64 || steps < 0) return ivr_fail(IVR_ERR_ARG, "samples %lld steps %d", samples, steps);
     int rc;
     for (int k = 0; k < steps; k++) {
@@ -847,6 +863,7 @@ int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, 
                        float *gtable_out, int gtable_brick)
 {
     if (!ctx || !s || n < 1 || !xyz || !bricks || !targets) return ivr_fail(IVR_ERR_ARG, "bad argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     for (int i = 0; i < n; i++)
         if (bricks[i] < 0 || bricks[i] >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", bricks[i]);
     const size_t bytes = (size_t)n * 4 * 6;
@@ -878,6 +895,7 @@ int ivr_surrogate_grad(ivr_ctx *ctx, ivr_surrogate *s, int n, const float *xyz, 
 int ivr_surrogate_adam(ivr_ctx *ctx, ivr_surrogate *s)
 {
     if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     int rc = adam_all(ctx, s);
     if (rc) return rc;
     IVR_HIP(hipStreamSynchronize(ctx->stream));
@@ -887,6 +905,7 @@ int ivr_surrogate_adam(ivr_ctx *ctx, ivr_surrogate *s)
 int ivr_surrogate_get(ivr_ctx *ctx, const ivr_surrogate *s, float *mlp, float *table, int brick)
 {
     if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     if (mlp) IVR_HIP(hipMemcpyAsync(mlp, s->params, SG_NPARAM * 4, hipMemcpyDeviceToHost, ctx->stream));
     if (table) {
         if (brick < 0 || brick >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", brick);
@@ -900,6 +919,7 @@ int ivr_surrogate_get(ivr_ctx *ctx, const ivr_surrogate *s, float *mlp, float *t
 int ivr_surrogate_set(ivr_ctx *ctx, ivr_surrogate *s, const float *mlp, const float *table, int brick)
 {
     if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     if (mlp) IVR_HIP(hipMemcpyAsync(s->params, mlp, SG_NPARAM * 4, hipMemcpyHostToDevice, ctx->stream));
     if (table) {
         if (brick < 0 || brick >= s->nbricks) return ivr_fail(IVR_ERR_ARG, "brick %d", brick);
@@ -932,8 +952,9 @@ struct SgRender {
     RenderArgs a;
     Enc enc;
     Mlp mlp;
-    const unsigned char *fallback; /* per brick: 1 = fetch the plain volume */
-    int only_brick;                /* >= 0: composite only that brick's samples */
+    const unsigned char *fallback; /* per surrogate brick: 1 = fetch the plain volume */
+    int only_brick;                /* >= 0: composite only that surrogate brick's samples */
+    int snb[3], sbshift;           /* surrogate bricks per axis, log2 of their edge */
     int skip;                      /* macrocell skipping (k_render_sg_items) */
     int nblocks;                   /* pixel-block workgroups of the frame */
 };
@@ -976,7 +997,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     unsigned cnt = 0, fetched = 0;
     bool live = n > 0;
     int k = 0;
-    const float invB = 1.0f / (float)(1 << a.bshift);
+    const float invB = 1.0f / (float)(1 << S.sbshift);
     float go[3], gd[3], rgd[3];
     ray_voxel(a, a.o, d, go, gd);
     if (SKIP) {
@@ -1005,13 +1026,23 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         }
         if (live) {
             if (!SKIP) p = cfg_pos(a, go, gd, tin, dt, k); /* with SKIP the walk above left it at k */
-            const int bx = p.j >> a.bshift, by = p.i >> a.bshift, bz = p.k >> a.bshift;
-            b = bx + a.nb[0] * (by + a.nb[1] * bz);
-            x[0] = ((float)p.j + p.fx - (float)(bx << a.bshift)) * invB;
-            x[1] = ((float)p.i + p.fy - (float)(by << a.bshift)) * invB;
-            x[2] = ((float)p.k + p.fz - (float)(bz << a.bshift)) * invB;
+            const int sh = S.sbshift;
+            const int bx = p.j >> sh, by = p.i >> sh, bz = p.k >> sh;
+            b = bx + S.snb[0] * (by + S.snb[1] * bz);
+            x[0] = ((float)p.j + p.fx - (float)(bx << sh)) * invB;
+            x[1] = ((float)p.i + p.fy - (float)(by << sh)) * invB;
+            x[2] = ((float)p.k + p.fz - (float)(bz << sh)) * invB;
         }
-        const float yv = sg_infer(S.enc, S.mlp, x, b);
+        /* The network runs only when a lane of the wavefront takes its value:
+         * wavefronts whose live samples all lie in fallback bricks (or outside
+         * only_brick) skip the encoding and the MLP.  Lanes that discard the value
+         * evaluate the centre of their brick, which keeps their gathers off the
+         * cache lines of samples that are not needed.  A lane's value depends on
+         * its own sample only, so this changes no result. */
+        const bool net = live && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
+        if (!net) x[0] = x[1] = x[2] = 0.5f;
+        float yv = 0.0f;
+        if (__ballot(net) != 0ull) yv = sg_infer(S.enc, S.mlp, x, b);
         if (live) {
             if (S.only_brick < 0 || S.only_brick == b) {
                 float s = yv;
@@ -1110,8 +1141,7 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
     if (d->state.winwidth != fr->W || d->state.winheight != fr->H)
         return ivr_fail(IVR_ERR_ARG, "state %dx%d does not match frame %dx%d", d->state.winwidth,
                         d->state.winheight, fr->W, fr->H);
-    if (vol && (vol->nbricks != s->nbricks || vol->B != s->B))
-        return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
+    if (vol && !ivr_surrogate_matches(s, vol)) return ivr_fail(IVR_ERR_ARG, "volume does not match surrogate");
     /* the geometry of the trained volume; a stand-in volume object carries it
      * when no plain data is resident */
     ivr_volume geo;
@@ -1122,7 +1152,7 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
             geo.nb[r] = s->nb[r];
         }
         geo.B = s->B;
-        geo.bshift = __builtin_ctz((unsigned)s->B);
+        geo.bshift = s->bshift;
         geo.P = s->B + 2;
         geo.nbricks = s->nbricks;
         geo.d_boff = nullptr;
@@ -1137,6 +1167,8 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
     S.mlp.p = s->params;
     S.fallback = fallback;
     S.only_brick = only_brick;
+    for (int r = 0; r < 3; r++) S.snb[r] = s->nb[r];
+    S.sbshift = s->bshift;
     for (int r = 0; r < 3; r++) S.a.nc[r] = s->nc[r];
     S.a.cellempty = cells;
     ivr_set_grid(S.a, ctx, fr->W, fr->H, IVR_GRID_SURROGATE);
@@ -1232,6 +1264,7 @@ void ivr_surrogate_defaults(ivr_surrogate_params *sp)
     sp->eps = 1e-15f;
     sp->init_scale = 1e-4f;
     sp->seed = 4004;
+    sp->log2_group = 0;
 }
 
 int ivr_surrogate_render(ivr_ctx *ctx, const ivr_surrogate *s, const ivr_volume *vol, const ivr_tf *tf,
@@ -1277,6 +1310,23 @@ int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned ch
     if (!ctx || !s || !flags) return ivr_fail(IVR_ERR_ARG, "null argument");
     IVR_HIP(hipMemcpyAsync(s->d_fallback, flags, s->nbricks, hipMemcpyHostToDevice, ctx->stream));
     IVR_HIP(hipStreamSynchronize(ctx->stream));
+    return IVR_OK;
+}
+
+int ivr_surrogate_freeze(ivr_ctx *ctx, ivr_surrogate *s)
+{
+    if (!ctx || !s) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return IVR_OK;
+    IVR_HIP(hipStreamSynchronize(ctx->stream)); /* queued work may still read the training state */
+    for (float **p : {&s->tables, &s->gtables, &s->mt, &s->vt}) {
+        ivr_dfree(ctx, *p, s->ntable * 4);
+        *p = nullptr;
+    }
+    for (float **p : {&s->gparams, &s->mp, &s->vp}) {
+        ivr_dfree(ctx, *p, SG_NPARAM * 4);
+        *p = nullptr;
+    }
+    s->frozen = true;
     return IVR_OK;
 }
 
@@ -1328,15 +1378,17 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
     return ivr_surrogate_set_fallback(ctx, s, flags.data());
 }
 
-/* checkpoint: "IVRSG002", params struct, dims, step, then MLP (params, m, v),
- * tables (values, m, v) and the macrocell ranges as little-endian float32
- * ("IVRSG001" files, without the ranges, still load) */
+/* checkpoint: "IVRSG003", params struct, dims, step, then MLP (params, m, v),
+ * tables (values, m, v) and the macrocell ranges as little-endian float32.
+ * "IVRSG002" files (a params struct without log2_group, which is then 0) and
+ * "IVRSG001" files (without the ranges too) still load. */
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
 {
     if (!ctx || !s || !path) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     FILE *f = fopen(path, "wb");
     if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
-    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '2'};
+    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '3'};
     int hdr[8] = {s->nbricks, s->B, s->n[0], s->n[1], s->n[2], s->step, (int)s->T, SG_NPARAM};
     bool ok = fwrite(magic, 8, 1, f) == 1 && fwrite(&s->sp, sizeof s->sp, 1, f) == 1 &&
               fwrite(hdr, sizeof hdr, 1, f) == 1;
@@ -1355,24 +1407,28 @@ int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
 {
     if (!ctx || !s || !path) return ivr_fail(IVR_ERR_ARG, "null argument");
+    if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     FILE *f = fopen(path, "rb");
     if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
     char magic[8];
     ivr_surrogate_params sp;
     int hdr[8];
-    bool ok = fread(magic, 8, 1, f) == 1 && (!memcmp(magic, "IVRSG001", 8) || !memcmp(magic, "IVRSG002", 8)) &&
-              fread(&sp, sizeof sp, 1, f) == 1 && fread(hdr, sizeof hdr, 1, f) == 1;
+    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG00", 7) && magic[7] >= '1' && magic[7] <= '3';
+    /* versions 1 and 2 stored the struct up to seed */
+    const size_t sp_bytes = ok && magic[7] < '3' ? offsetof(ivr_surrogate_params, log2_group) : sizeof sp;
+    memset(&sp, 0, sizeof sp);
+    ok = ok && fread(&sp, sp_bytes, 1, f) == 1 && fread(hdr, sizeof hdr, 1, f) == 1;
     if (!ok) {
         fclose(f);
         return ivr_fail(IVR_ERR_IO, "%s is not a surrogate checkpoint", path);
     }
-    const int nsec = magic[7] == '2' ? 7 : 6;
+    const int nsec = magic[7] >= '2' ? 7 : 6;
     /* the table layout (level resolutions, dense/hashed split, slots) is
      * fixed at create by log2_table, base_res and growth: a checkpoint made
      * with other values would load without error and render wrong values */
     if (hdr[0] != s->nbricks || hdr[1] != s->B || hdr[2] != s->n[0] || hdr[3] != s->n[1] || hdr[4] != s->n[2] ||
         hdr[6] != (int)s->T || hdr[7] != SG_NPARAM || sp.log2_table != s->sp.log2_table ||
-        sp.base_res != s->sp.base_res || sp.growth != s->sp.growth) {
+        sp.base_res != s->sp.base_res || sp.growth != s->sp.growth || sp.log2_group != s->sp.log2_group) {
         fclose(f);
         return ivr_fail(IVR_ERR_ARG, "checkpoint %s does not match this surrogate (bricks, dims, table size or "
                                      "level resolutions differ)", path);

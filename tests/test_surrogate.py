@@ -357,3 +357,71 @@ def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
     sg.close()
     fr.close()
     tf.close()
+
+
+@pytest.mark.gpu
+def test_grouped_bricks_and_freeze(ctx, small_vol):
+    """A surrogate whose bricks span 2x2x2 volume bricks (log2_group=1) over the
+    64^3 volume in 32^3 bricks is the surrogate of the same volume in one 64^3
+    brick: the same training samples (equal first-step loss from equal
+    initial weights) and, with the same weights, bit-identical frames.  Forced
+    fallback through the grouped geometry equals the plain render.  Freezing
+    releases 16 bytes per table parameter and changes no frame."""
+    vol32, host = small_vol
+    vol64 = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
+    vol64.upload(host)
+    sg_g = ctx.surrogate(vol32, log2_table=12, log2_group=1, seed=77)
+    sg_1 = ctx.surrogate(vol64, log2_table=12, seed=77)
+    assert sg_g.nbricks == 1 and sg_1.nbricks == 1
+    with pytest.raises(ivr.IvrError):
+        sg_g.train(vol64, 1, 1 << 12)  # a volume of other bricks than the surrogate was made for
+    lg, l1 = sg_g.train(vol32, 1, 1 << 16), sg_1.train(vol64, 1, 1 << 16)
+    print("first-step loss grouped %.6e, one brick %.6e" % (lg, l1))
+    assert abs(lg - l1) <= 1e-4 * l1
+    sg_g.train(vol32, 20, 1 << 16)
+    mlp, tb = sg_g.get(0)
+    sg_1.set(mlp=mlp, table=tb, brick=0)
+    tf = ctx.tf(configs.tf_lut(2))
+    st = configs.state(2, 160, 144)
+    fr = ctx.frame(160, 144)
+    frames = {}
+    for skip in (False, True):
+        d = ivr.config_desc(st, skip=skip)
+        sg_g.render(vol32, tf, d, fr)
+        a = fr.download()
+        sg_1.render(vol64, tf, d, fr)
+        assert np.array_equal(a, fr.download())
+        frames[skip] = a
+    d = ivr.config_desc(st)
+    ctx.render(vol32, tf, d, fr)
+    plain = fr.download()
+    print("grouped surrogate PSNR %.2f dB" % ivr.psnr(frames[False], plain))
+    sg_g.set_fallback(np.ones(1, np.uint8))
+    sg_g.render(vol32, tf, d, fr)
+    assert np.array_equal(fr.download(), plain)
+    sg_g.set_fallback(np.zeros(1, np.uint8))
+    # freeze: the training state goes, the frames stay
+    xyz = np.random.default_rng(5).uniform(0, 1, (100, 3)).astype(np.float32)
+    y0 = sg_g.eval_points(xyz, np.zeros(100, np.int32))
+    b0 = ctx.device_bytes()
+    sg_g.freeze()
+    freed = b0 - ctx.device_bytes()
+    assert freed == sg_g.nbricks * 16 * sg_g.T * 2 * 16 + 3 * ivr.SG_NPARAM * 4
+    sg_g.render(vol32, tf, d, fr)
+    assert np.array_equal(fr.download(), frames[False])
+    sg_g.render(None, tf, d, fr)  # no plain volume resident
+    assert np.array_equal(fr.download(), frames[False])
+    assert np.array_equal(sg_g.eval_points(xyz, np.zeros(100, np.int32)), y0)
+    for call in (lambda: sg_g.train(vol32, 1, 1 << 12), lambda: sg_g.get(0),
+                 lambda: sg_g.save(os.path.join(tempfile.mkdtemp(), "x"))):
+        with pytest.raises(ivr.IvrError):
+            call()
+    sg_g.freeze()  # a second freeze is a no-op
+    # a checkpoint of the one-brick surrogate does not load into the grouped one
+    path = os.path.join(tempfile.mkdtemp(), "sg1.ckpt")
+    sg_1.save(path)
+    sg_h = ctx.surrogate(vol32, log2_table=12, log2_group=1)
+    with pytest.raises(ivr.IvrError):
+        sg_h.load(path)
+    for o in (sg_h, sg_g, sg_1, fr, tf, vol64):
+        o.close()
