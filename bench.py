@@ -71,6 +71,9 @@ def main():
                     help="the C2 frame rate alone, without the survey-TF, surrogate and serving parts")
     ap.add_argument("--sg-volume", type=int, default=2, choices=[2, 3],
                     help="--config 4: train on the C2 (512^3 fp32) or the C3 (1024^3 fp16) volume")
+    ap.add_argument("--sg-group", type=int, default=0, choices=[0, 1, 2, 3, 4],
+                    help="--config 4: one hash grid per 2^g volume bricks per axis (g > 0: frozen compact "
+                         "surrogate, compared with the plain field at the same bytes)")
     ap.add_argument("--serve-surrogate", action="store_true",
                     help="--config 5: also serve the 1024^3 volume from a surrogate trained for --train-seconds")
     ap.add_argument("--requests", type=int, default=400, help="--config 5: requests per arrival rate")
@@ -243,7 +246,8 @@ def main():
         cfg3 = configs.WORKLOADS[3]
         v3 = ctx.volume(*cfg3["dims"], cfg3["dtype"], brick=cfg3["brick"])
         v3.generate(configs.gen_params(3))
-        sur["vol_1024"], sg3 = surrogate_measure(ctx, v3, 3, args.train_seconds, args.steps, keep=True)
+        # the 1024^3 surrogate in grids of 512^3 (2x2x2 volume bricks), frozen: 1/8 of the fp16 field
+        sur["vol_1024"], sg3 = surrogate_measure(ctx, v3, 3, args.train_seconds, args.steps, keep=True, group=1)
         out["surrogate"] = sur
         vols = serving_volumes(ctx, c2=vol, c3=v3)
         n_req = 200
@@ -324,9 +328,9 @@ def _frame_ms(ctx, fn, steps):
     return ctx.elapsed(0, 1) / steps
 
 
-def train_surrogate(ctx, vol, seconds, batch=1 << 24, chunk=5):
+def train_surrogate(ctx, vol, seconds, batch=1 << 24, chunk=5, group=0):
     """Adam steps of `batch` samples in chunks until `seconds` of training"""
-    sg = ctx.surrogate(vol)
+    sg = ctx.surrogate(vol, log2_group=group)
     losses, steps, t_train = [], 0, 0.0
     while t_train < seconds:
         ctx.sync()
@@ -339,13 +343,46 @@ def train_surrogate(ctx, vol, seconds, batch=1 << 24, chunk=5):
     return sg, losses, steps, t_train
 
 
-def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=False):
+def lowres_field(vol, f):
+    """the plain method at a surrogate's footprint: vol's values at every f-th
+    grid point per axis, rounded to fp16 (1/f^3 of the fp16 field's bytes), and
+    the full grid reconstructed from them by trilinear interpolation (host
+    array, float32)"""
+    import numpy as np
+    a = vol.download()
+    shape = a.shape
+    g = a[::f, ::f, ::f].astype(np.float16).astype(np.float32)
+    del a
+    for axis in range(3):
+        n, K = shape[axis], g.shape[axis]
+        i = np.arange(n)
+        k = np.minimum(i // f, K - 1)
+        k1 = np.minimum(k + 1, K - 1)
+        bshape = [1, 1, 1]
+        bshape[axis] = n
+        w = ((i - k * f) / f).astype(np.float32).reshape(bshape)
+        out = np.take(g, k, axis)
+        out *= 1.0 - w
+        hi = np.take(g, k1, axis)
+        hi *= w
+        out += hi
+        del hi
+        g = out
+    return g
+
+
+def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=False, group=0):
     """A surrogate of `vol` (config c's volume: 2 = the 512^3 fp32 C2 volume,
-    3 = the 1024^3 fp16 C3 volume): per-brick hash grids (16 levels x 2^19 x
-    2 features) + the BF16 MFMA MLP trained on the GPU with 2^24 samples per
-    Adam step for a time budget, then 1920x1080 frames from it against the
-    plain render of the same camera: PSNR/SSIM, frames/s, per-brick fallback
-    below 40 dB.  Returns (dict, surrogate or None)."""
+    3 = the 1024^3 fp16 C3 volume): hash grids (16 levels x 2^19 x 2
+    features) per brick of 2^group volume bricks per axis + the BF16 MFMA MLP
+    trained on the GPU with 2^24 samples per Adam step for a time budget, then
+    1920x1080 frames from it against the plain render of the same camera:
+    PSNR/SSIM, frames/s, per-brick fallback below 40 dB.  With group > 0 the
+    surrogate is frozen to its half-precision tables and rendered without the
+    plain volume (unless a brick falls back), its resident bytes are reported
+    beside the field's, and the plain method at the same bytes (the field at
+    1/2^group resolution in fp16, trilinearly reconstructed) is rendered for
+    comparison.  Returns (dict, surrogate or None)."""
     import numpy as np
     import ivr_amd as ivr
     from ivr_amd import configs
@@ -356,7 +393,9 @@ def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=F
     plain_ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, desc, fr), steps)
     plain = fr.download()
     batch = 1 << 24
-    sg, losses, nsteps, t_train = train_surrogate(ctx, vol, train_seconds, batch)
+    b0 = ctx.device_bytes()
+    sg, losses, nsteps, t_train = train_surrogate(ctx, vol, train_seconds, batch, group=group)
+    train_bytes = ctx.device_bytes() - b0
     step_ms = t_train * 1e3 / nsteps
     sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
     sg_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), steps)
@@ -364,20 +403,46 @@ def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=F
     img = fr.download()
     p_sg, s_sg = ivr.psnr(img, plain), ivr.ssim(img, plain)
     bpsnr, nfall = sg.evaluate(vol, tf, desc, threshold_db=40.0)
-    fb_ms = _frame_ms(ctx, lambda: sg.render(vol, tf, desc, fr), steps)
+    cpu_sg = cpu_surrogate_baseline(sg, cpu_seconds) if cpu_seconds > 0 else None
+    compact = None
+    rvol = vol
+    if group > 0:
+        sg.freeze()
+        rvol = vol if nfall else None  # the plain volume only for bricks that fall back
+        compact = {"log2_group": group, "surrogate_bricks": sg.nbricks,
+                   "resident_bytes_training": int(train_bytes),
+                   "resident_bytes_frozen": int(ctx.device_bytes() - b0),
+                   "field_bytes": int(vol.info()["bytes"]), "rendered_without_plain_volume": rvol is None}
+    fb_ms = _frame_ms(ctx, lambda: sg.render(rvol, tf, desc, fr), steps)
     img_fb = fr.download()
     p_fb, s_fb = ivr.psnr(img_fb, plain), ivr.ssim(img_fb, plain)
     # the network kernel's own duration (HIP events around k_render_sg on the library's stream)
-    k_ms, _ = kernel_ms(ctx, lambda: sg.render(vol, tf, desc, fr), fr, steps)
+    k_ms, _ = kernel_ms(ctx, lambda: sg.render(rvol, tf, desc, fr), fr, steps)
     S = int(fr.stats()["visited"])
+    if compact:
+        f = 1 << group
+        low = ctx.volume(*vol.shape[::-1], vol.info()["dtype"], brick=vol.info()["brick"])
+        low.upload(lowres_field(vol, f))
+        ctx.render(low, tf, desc, fr)
+        img_low = fr.download()
+        low.close()
+        compact["frozen_over_field"] = round(compact["resident_bytes_frozen"] / compact["field_bytes"], 4)
+        compact["equal_bytes_plain"] = {
+            "method": "field at every %d-th voxel per axis in fp16 (%d bytes), trilinear reconstruction" % (
+                f, 2 * int(np.prod([(n + f - 1) // f for n in vol.shape]))),
+            "psnr_db": round(ivr.psnr(img_low, plain), 2), "ssim": round(ivr.ssim(img_low, plain), 4)}
+        log("compact surrogate: %d bytes frozen (%.4f of the field), PSNR %.2f dB; plain field at 1/%d "
+            "resolution: %.2f dB" % (compact["resident_bytes_frozen"], compact["frozen_over_field"], p_fb, f,
+                                     compact["equal_bytes_plain"]["psnr_db"]))
     log("surrogate %s: train %.1f ms/step (%d steps), render %.3f ms (plain %.3f); PSNR %.2f dB SSIM %.4f; "
         "%d/%d bricks below 40 dB -> with fallback PSNR %.2f dB, %.3f ms"
         % ("C2" if c == 2 else "1024^3", step_ms, nsteps, sg_ms, plain_ms, p_sg, s_sg, nfall, sg.nbricks, p_fb,
            fb_ms))
     flops = 2 * (32 * 64 + 64 * 64 + 64)
     g_bytes = S * 16 * 8 * 4  # per network sample: 16 levels x 8 corners x 2 half-precision features
-    workload = "surrogate of %s: %d bricks x 16 levels x 2^19 x 2 features, MLP 32-64-64-1, 1920x1080" \
-        % ("C2 (512^3 fp32)" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks)
+    workload = "surrogate of %s: %d bricks%s x 16 levels x 2^19 x 2 features, MLP 32-64-64-1, 1920x1080" \
+        % ("C2 (512^3 fp32)" if c == 2 else "C3 (1024^3 fp16)", sg.nbricks,
+           " of %d^3" % (vol.info()["brick"] << group) if group else "")
     out = {
         "workload": workload,
         "fps": round(1000.0 / fb_ms, 2), "ms_per_frame": round(fb_ms, 3),
@@ -400,8 +465,10 @@ def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=F
                      "mfma_tflops": round(S * flops / (k_ms * 1e-3) / 1e12, 2), "mfma_peak_tflops": 2500.0,
                      "mfma_frac": round(S * flops / (k_ms * 1e-3) / 2.5e15, 5)},
     }
-    if cpu_seconds > 0:
-        out["cpu_baseline"] = cpu_surrogate_baseline(sg, cpu_seconds)
+    if compact:
+        out["compact"] = compact
+    if cpu_sg:
+        out["cpu_baseline"] = cpu_sg
     fr.close()
     tf.close()
     if not keep:
@@ -420,7 +487,8 @@ def bench_surrogate(args):
     ctx = ivr.Context(0)
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(c))
-    res, _ = surrogate_measure(ctx, vol, c, args.train_seconds, args.steps, 0.0 if args.no_cpu else args.cpu_seconds)
+    res, _ = surrogate_measure(ctx, vol, c, args.train_seconds, args.steps, 0.0 if args.no_cpu else args.cpu_seconds,
+                               group=args.sg_group)
     out = {
         "metric": "surrogate frames/s at 1920x1080 with per-brick fallback below 40 dB",
         "value": res["fps"],
