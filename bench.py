@@ -246,6 +246,7 @@ def main():
         cfg3 = configs.WORKLOADS[3]
         v3 = ctx.volume(*cfg3["dims"], cfg3["dtype"], brick=cfg3["brick"])
         v3.generate(configs.gen_params(3))
+        out["c3"] = c3_measure(ctx, v3, args.steps)
         # the 1024^3 surrogate in grids of 512^3 (2x2x2 volume bricks), frozen: 1/8 of the fp16 field
         sur["vol_1024"], sg3 = surrogate_measure(ctx, v3, 3, args.train_seconds, args.steps, keep=True, group=1)
         out["surrogate"] = sur
@@ -268,6 +269,47 @@ def main():
     print(json.dumps(out), flush=True)
     vol.close()
     ctx.close()
+
+
+def c3_measure(ctx, vol, steps):
+    """The C3 frame on a resident C3 volume (1024^3 fp16, 4x4x4 bricks of
+    256^3, macrocell skipping, sort-last, 1920x1080): ms/frame, Gsamples/s,
+    the k_runs roofline, and the GPU image against the oracle over a band of
+    rows through the centre"""
+    import numpy as np
+    import ivr_amd as ivr
+    import oracle
+    from ivr_amd import configs
+    cfg = configs.WORKLOADS[3]
+    W, H = cfg["image"]
+    lut = configs.tf_lut(3)
+    tf = ctx.tf(lut)
+    st = configs.state(3)
+    d = ivr.config_desc(st, skip=True, composite=ivr.SORT_LAST)
+    fr = ctx.frame(W, H)
+    ms = _frame_ms(ctx, lambda: ctx.render(vol, tf, d, fr), steps)
+    stats = fr.stats()
+    S = int(stats["visited"])
+    img = fr.download()
+    k_ms, c_ms = kernel_ms(ctx, lambda: ctx.render(vol, tf, d, fr), fr, steps)
+    # 8 fp16 voxels per classified sample, the partials the rays have, a run count per pixel
+    bytes_alg = S * 16 + int(stats["partials"]) * 16 + 4 * W * H
+    r0, r1 = H // 2 - 8, H // 2 + 8
+    ref32, _ = _oracle_frame(oracle, vol.download(), st, lut, ivr.SORT_LAST, cfg["brick"], (r0, r1), cpu_threads())
+    dmax = int(np.abs(oracle.quantize(ref32[r0:r1]).astype(int) - img[r0:r1].astype(int)).max())
+    fr.close()
+    tf.close()
+    log("C3: %.4f ms/frame, %d samples classified, %.1f Gsamples/s, k_runs %.4f ms; max |gpu - oracle| %d/255 "
+        "on rows %d-%d" % (ms, S, S / ms / 1e6, k_ms, dmax, r0, r1))
+    return {"workload": cfg["name"], "ms_per_frame": round(ms, 4), "frames_per_s": round(1000.0 / ms, 2),
+            "samples_per_frame": S, "samples_incl_skipped": int(stats["samples"]),
+            "gsamples_s": round(S / ms / 1e6, 2),
+            "roofline": {"bound": "hbm", "kernel": "k_runs<_Float16, true, true>", "kernel_ms": round(k_ms, 4),
+                         "combine_ms": round(c_ms, 4), "algorithmic_bytes": bytes_alg,
+                         "achieved": round(bytes_alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": round(bytes_alg / (k_ms * 1e-3) / HBM_PEAK, 4),
+                         "traffic": measured_traffic(cfg["name"], "sort_last", True)},
+            "max_abs_diff_vs_oracle": dmax, "oracle_rows": [r0, r1]}
 
 
 def survey_tf_measure(ctx, vol, st, comp, W, H, brick, steps):
