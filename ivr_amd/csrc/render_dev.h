@@ -130,20 +130,25 @@ __device__ __forceinline__ void ray_voxel(const RenderArgs &a, const float *o, c
 
 __device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
 {
-    return (p.j >> IVR_CELL_SHIFT) +
-           a.nc[0] * ((p.i >> IVR_CELL_SHIFT) + a.nc[1] * (p.k >> IVR_CELL_SHIFT));
+    /* on the full-rate 24-bit multiplier: the cell counts per axis and the
+     * partial sums are below 2^24, the index below 2^32 */
+    return (int)((unsigned)(p.j >> IVR_CELL_SHIFT) +
+                 __umul24((unsigned)a.nc[0], (unsigned)(p.i >> IVR_CELL_SHIFT) +
+                                                 __umul24((unsigned)a.nc[1], (unsigned)(p.k >> IVR_CELL_SHIFT))));
 }
 
 /* First sample index past k on the grid t_k = tin + k dt whose cell may be
  * visible, given that sample k at p has the nonzero transparency code e
  * (its node of 2^(e-1) cells per axis is transparent): the sample just
- * before the ray's exit from that node, estimated from the exit faces, is
- * checked to lie in the node, else the walk advances by one sample.  Each
- * axis index of the samples is monotone in k, so all samples between lie
- * in the node.  The result is at most kmax. */
+ * before the ray's exit from that node, estimated from the exit faces (with
+ * rdt ~ 1/dt), is checked to lie in the node, else the walk advances by one
+ * sample.  Each axis index of the samples is monotone in k, so all samples
+ * between lie in the node.  The result is at most kmax.  Only samples of
+ * transparent cells are passed, whatever the estimate: it decides how many
+ * steps a skip takes, never which samples are composited. */
 __device__ __forceinline__ int skip_empty(const RenderArgs &a, const float *go, const float *gd,
-                                          const float *rgd, float tin, float dt, const Pos &p, int e,
-                                          int k, int kmax)
+                                          const float *rgd, float tin, float dt, float rdt, const Pos &p,
+                                          int e, int k, int kmax)
 {
     const int sh = IVR_CELL_SHIFT - 1 + e;
     const int cc[3] = {p.j >> sh, p.i >> sh, p.k >> sh};
@@ -156,7 +161,7 @@ __device__ __forceinline__ int skip_empty(const RenderArgs &a, const float *go, 
         }
     }
     int kc = k + 1;
-    const float qk = (te - tin) / dt;
+    const float qk = (te - tin) * rdt;
     if (qk > (float)(k + 1)) {
         const int cand = qk < (float)kmax ? (int)qk : kmax;
         if (cand > k + 1) {
@@ -196,7 +201,7 @@ __device__ __forceinline__ void ftb_sample(const RenderArgs &a, const float4 *s_
 
 /* One config-mode ray on the global sample grid t_k = tin + k dt, k < n. */
 struct Ray {
-    float tin, dt;
+    float tin, dt, rdt; /* rdt = 1/dt, for estimates that are checked exactly */
     int n;
     float go[3], gd[3], rgd[3]; /* voxel-space ray g(t) = go + gd t */
 };
@@ -210,6 +215,7 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs &a, const float *o, c
     r.tin = fmaxf(tin, 0.0f);
     const float dlen = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
     r.dt = (a.vox * a.step_vox) / dlen;
+    r.rdt = 1.0f / r.dt;
     const float nf = 1.0f + (tout - r.tin) / r.dt + 0.5f;
     /* guard for degenerate directions; never reached by a finite ray */
     r.n = nf < (float)IVR_MAX_STEPS ? (int)nf : IVR_MAX_STEPS;
@@ -246,6 +252,16 @@ __device__ __forceinline__ void fetch8_buf(__amdgpu_buffer_rsrc_t rs, unsigned o
     }
 }
 
+/* byte offset in the volume of the footprint origin of sample position q
+ * (the buffer form of fetch8's address; the volume is below 4 GiB) */
+template <typename VT>
+__device__ __forceinline__ unsigned vox_offset(const RenderArgs &a, const unsigned long long *s_boff, const Pos &q)
+{
+    const int bid = (q.j >> a.bshift) + a.nb[0] * ((q.i >> a.bshift) + a.nb[1] * (q.k >> a.bshift));
+    return (unsigned)(s_boff[bid] + brick_elem(a.P, (q.j & a.bmask) + 1, (q.i & a.bmask) + 1, (q.k & a.bmask) + 1)) *
+           (unsigned)sizeof(VT);
+}
+
 /* march_segment without macrocell skipping, software-pipelined: the next
  * sample's footprint is requested before the current sample is composited,
  * and a lane whose next sample shares its floor index requests one past the
@@ -262,11 +278,7 @@ __device__ __forceinline__ void march_segment_pipe(const RenderArgs &a, const fl
     const __amdgpu_buffer_rsrc_t rs =
         __builtin_amdgcn_make_buffer_rsrc((void *)a.vox_data, (short)0, (int)a.vox_bytes, 0x00020000);
     const unsigned esz = sizeof(VT), cy = (unsigned)a.P * esz, cz = (unsigned)(a.P * a.P) * esz;
-    auto offset = [&](const Pos &q) -> unsigned {
-        const int bid = (q.j >> a.bshift) + a.nb[0] * ((q.i >> a.bshift) + a.nb[1] * (q.k >> a.bshift));
-        return (unsigned)(s_boff[bid] + brick_elem(a.P, (q.j & a.bmask) + 1, (q.i & a.bmask) + 1,
-                                                    (q.k & a.bmask) + 1)) * esz;
-    };
+    auto offset = [&](const Pos &q) { return vox_offset<VT>(a, s_boff, q); };
     int k = k0;
     Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
     float v[8];
@@ -303,6 +315,12 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
         march_segment_pipe<VT>(a, s_lut, s_boff, r, k0, k1, acc, cnt, fetched);
         return;
     }
+    /* footprints through the buffer resource (two voxels per load) when the
+     * volume allows it (uniform per launch), else through plain pointers */
+    const bool buf = a.vox_bytes != 0;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.vox_data, (short)0, (int)a.vox_bytes, 0x00020000);
+    const unsigned esz = sizeof(VT), cy = (unsigned)a.P * esz, cz = (unsigned)(a.P * a.P) * esz;
     int k = k0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     int cc = -1, e = 0;            /* macrocell of the code held in e      */
@@ -316,14 +334,15 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
                 e = a.cellempty[cell];
             }
             if (e) { /* transparent node: jump past it */
-                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, p, e, k, k1);
+                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, r.rdt, p, e, k, k1);
                 cnt += (unsigned)(kc - k);
                 k = kc;
                 continue;
             }
         }
         if (p.j != cj || p.i != ci || p.k != ck) { /* consecutive samples often share their 8 voxels */
-            fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+            if (buf) fetch8_buf<VT>(rs, vox_offset<VT>(a, s_boff, p), cy, cz, v);
+            else fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
             cj = p.j;
             ci = p.i;
             ck = p.k;
@@ -371,7 +390,7 @@ __device__ __forceinline__ int run_end(const RenderArgs &a, const Ray &r, int k,
         te = fminf(te, (face - r.go[q]) * r.rgd[q]);
     }
     int e = r.n;
-    const float qk = ceilf((te - r.tin) / r.dt);
+    const float qk = ceilf((te - r.tin) * r.rdt);
     if (qk < (float)r.n) e = max(k + 1, (int)qk);
     auto owned = [&](int kk) { return brick_of(a, cfg_pos(a, r.go, r.gd, r.tin, r.dt, kk)) == b; };
     int lo, hi; /* owned(lo), and hi == r.n or !owned(hi) */
@@ -453,7 +472,7 @@ __device__ __forceinline__ void march_sort_last(const RenderArgs &a, const float
                 /* transparent node: jump past it.  Its samples add exact
                  * zeros to the run's partial, and a run skipped whole has an
                  * all-zero partial, which `over` leaves without effect. */
-                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, p, e, k, r.n);
+                const int kc = skip_empty(a, r.go, r.gd, r.rgd, r.tin, r.dt, r.rdt, p, e, k, r.n);
                 cnt += (unsigned)(kc - k);
                 k = kc;
                 continue;

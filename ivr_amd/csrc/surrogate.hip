@@ -998,11 +998,12 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     bool live = n > 0;
     int k = 0;
     const float invB = 1.0f / (float)(1 << S.sbshift);
-    float go[3], gd[3], rgd[3];
+    float go[3], gd[3], rgd[3], rdt = 0.0f;
     ray_voxel(a, a.o, d, go, gd);
     if (SKIP) {
 #pragma unroll
         for (int q = 0; q < 3; q++) rgd[q] = 1.0f / gd[q];
+        rdt = 1.0f / dt;
     }
     int cc = -1, e = 0; /* macrocell of the transparency code held in e */
     while (__ballot(live) != 0ull) {
@@ -1018,7 +1019,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
                     e = a.cellempty[cell];
                 }
                 if (!e) break;
-                const int kc = skip_empty(a, go, gd, rgd, tin, dt, p, e, k, n);
+                const int kc = skip_empty(a, go, gd, rgd, tin, dt, rdt, p, e, k, n);
                 cnt += (unsigned)(kc - k);
                 k = kc;
             }
