@@ -58,8 +58,7 @@ __host__ __device__ __forceinline__ int kperm(int ks, int g, int j) { return (2 
 /* ---------------------------------------------------------------- encoding */
 
 struct Enc {
-    const float *table;    /* [nbricks][L][T][F] */
-    const __half2 *table16; /* the same in half precision (rendering), or null */
+    const __half2 *table16; /* [nbricks][L][T] feature pairs, the half-precision copy of the tables */
     unsigned T;
     int res[SG_L];
     unsigned n1[SG_L];  /* N+1 for dense levels, 0 for hashed */
@@ -115,26 +114,23 @@ __device__ __forceinline__ void level_cell(const Lvl &L, const float *x, int *v0
     }
 }
 
-__device__ __forceinline__ float2 table_at(const float2 *tb, unsigned i) { return tb[i]; }
-__device__ __forceinline__ float2 table_at(const __half2 *tb, unsigned i) { return __half22float2(tb[i]); }
-
 /* level L of the encoding at local position x: trilinear blend of the 8
- * corner features, from the fp32 tables (TT = float2, training) or their
- * half-precision copy (TT = __half2, rendering: half the bytes per corner) */
-template <typename TT = float2>
+ * corner features of the half-precision copy of the tables (training and
+ * rendering both read it: half the bytes per corner of the fp32 master
+ * tables, which only the optimiser reads) */
 __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lvl &L, const float *x)
 {
     int v0[3];
     float w[3];
     level_cell(L, x, v0, w);
     const size_t base = ((size_t)brick * SG_L + L.l) * (size_t)e.T;
-    const TT *tb = sizeof(TT) == sizeof(float2) ? (const TT *)e.table + base : (const TT *)e.table16 + base;
+    const __half2 *tb = e.table16 + base;
     float2 f = make_float2(0.0f, 0.0f);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
         const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
         const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const float2 t = table_at(tb, corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz));
+        const float2 t = __half22float2(tb[corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz)]);
         f.x = fmaf(wc, t.x, f.x);
         f.y = fmaf(wc, t.y, f.y);
     }
@@ -160,7 +156,6 @@ __device__ __forceinline__ void gather_samples(const float *x, int b, SampleSet 
     }
 }
 
-template <typename TT = float2>
 __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, bf16x8 E[4])
 {
     const int g = (threadIdx.x & 63) >> 4;
@@ -168,7 +163,7 @@ __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, b
     for (int nt = 0; nt < 4; nt++) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float2 f = encode_level<TT>(e, s.b[nt], lvl_of(e, g, q, 4), s.x[nt]);
+            const float2 f = encode_level(e, s.b[nt], lvl_of(e, g, q, 4), s.x[nt]);
             E[nt][2 * q] = (__bf16)f.x;
             E[nt][2 * q + 1] = (__bf16)f.y;
         }
@@ -304,7 +299,7 @@ __device__ __forceinline__ float sg_infer(const Enc &enc, const Mlp &mlp, const 
     SampleSet ss;
     gather_samples(x, b, ss);
     bf16x8 E[4];
-    encode_frags<__half2>(enc, ss, E);
+    encode_frags(enc, ss, E);
     float y[4];
     mlp_infer(mlp, E, y);
     return own(y);
@@ -406,6 +401,9 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
         }
         SampleSet ss;
         gather_samples(x, b, ss);
+        /* the forward pass reads the half-precision copy of the tables, as rendering
+         * does (the fp32 tables are the optimiser's master copy): half the gather bytes,
+         * and the network trained is the one rendered */
         bf16x8 E[4];
         encode_frags(A.enc, ss, E);
         Fwd f;
@@ -646,7 +644,6 @@ namespace {
 Enc make_enc(const ivr_surrogate *s)
 {
     Enc e;
-    e.table = s->tables;
     e.table16 = (const __half2 *)s->tables16;
     e.T = s->T;
     for (int l = 0; l < SG_L; l++) {

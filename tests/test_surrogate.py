@@ -264,6 +264,8 @@ def test_gpu_gradients_match_oracle(ctx, small_vol):
     bricks = rng.integers(0, 2, n).astype(np.int32)
     t = rng.uniform(0, 1, n).astype(np.float32)
     y, loss, g, gt0 = sg.grad(xyz, bricks, t, table_brick=0)
+    # the training forward pass reads the half-precision copy of the tables
+    tables = [tb.astype(np.float16).astype(np.float32) for tb in tables]
     # oracle: both bricks' points, MLP gradient summed; table gradient of brick 0
     yo = np.zeros(n)
     go = np.zeros(oracle.SG_NPARAM)
