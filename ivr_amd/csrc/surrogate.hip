@@ -156,6 +156,9 @@ __device__ __forceinline__ void gather_samples(const float *x, int b, SampleSet 
     }
 }
 
+/* SERIAL: the gathers of one sample column at a time (a scheduling barrier
+ * after each), which the register-bound training kernel needs */
+template <bool SERIAL = false>
 __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, bf16x8 E[4])
 {
     const int g = (threadIdx.x & 63) >> 4;
@@ -167,6 +170,7 @@ __device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, b
             E[nt][2 * q] = (__bf16)f.x;
             E[nt][2 * q + 1] = (__bf16)f.y;
         }
+        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -176,6 +180,34 @@ struct Mlp {
     const bf16x8 *frag; /* [SG_NFRAG][64] */
     const float *p;     /* fp32 parameters */
 };
+
+/* What inference reads of the network, staged in LDS by each workgroup: the
+ * A fragments of W1 and W2 (12 sets of 64 lanes) and the fp32 b1, b2, W3, b3.
+ * Read from LDS at each evaluation, they are not held in registers across a
+ * march (the render kernel's registers bound its occupancy). */
+#define SG_INFER_FRAGS (SG_FRAG_G2 - SG_FRAG_F1)
+#define SG_INFER_LDS (SG_INFER_FRAGS * 64 * 16 + (3 * SG_H + 1) * 4)
+struct MlpLds {
+    const bf16x8 *frag; /* [SG_INFER_FRAGS][64], sets numbered as in Mlp::frag */
+    const float *b1, *b2, *w3, *b3;
+};
+
+/* stage m into lds (SG_INFER_LDS bytes, 16-byte aligned); the caller synchronises */
+__device__ __forceinline__ MlpLds stage_mlp(const Mlp &m, void *lds)
+{
+    bf16x8 *f = (bf16x8 *)lds;
+    float *q = (float *)(f + SG_INFER_FRAGS * 64);
+    for (int t = threadIdx.x; t < SG_INFER_FRAGS * 64; t += blockDim.x) f[t] = m.frag[SG_FRAG_F1 * 64 + t];
+    for (int t = threadIdx.x; t < 3 * SG_H + 1; t += blockDim.x)
+        q[t] = t < SG_H ? m.p[SG_P_B1 + t] : t < 2 * SG_H ? m.p[SG_P_B2 + t - SG_H] : t < 3 * SG_H ? m.p[SG_P_W3 + t - 2 * SG_H] : m.p[SG_P_B3];
+    MlpLds v;
+    v.frag = f;
+    v.b1 = q;
+    v.b2 = q + SG_H;
+    v.w3 = q + 2 * SG_H;
+    v.b3 = q + 3 * SG_H;
+    return v;
+}
 
 /* forward of 64 samples: returns y for samples nt*16+l%16 in y[nt]; keeps
  * the first-layer activations (bf16 fragments) and second-layer
@@ -240,20 +272,20 @@ __device__ __forceinline__ void mlp_forward(const Mlp &m, const bf16x8 E[4], Fwd
  * each nt), but each layer-2 output tile is folded into y as soon as it is
  * made instead of being kept for a backward pass, which leaves 64 fewer
  * registers live across the product. */
-__device__ __forceinline__ void mlp_infer(const Mlp &m, const bf16x8 E[4], float y[4])
+__device__ __forceinline__ void mlp_infer(const MlpLds &m, const bf16x8 E[4], float y[4])
 {
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
     bf16x8 H[2][4];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-        const bf16x8 a = m.frag[(SG_FRAG_F1 + mt) * 64 + lane];
+        const bf16x8 a = m.frag[mt * 64 + lane]; /* the staged sets start at SG_FRAG_F1 */
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             f32x4 c = mfma(a, E[nt], zero);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const float z = c[i] + m.p[SG_P_B1 + mt * 16 + g * 4 + i];
+                const float z = c[i] + m.b1[mt * 16 + g * 4 + i];
                 H[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
             }
         }
@@ -261,16 +293,16 @@ __device__ __forceinline__ void mlp_infer(const Mlp &m, const bf16x8 E[4], float
     float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
-        const bf16x8 a0 = m.frag[(SG_FRAG_F2 + mt * 2 + 0) * 64 + lane];
-        const bf16x8 a1 = m.frag[(SG_FRAG_F2 + mt * 2 + 1) * 64 + lane];
+        const bf16x8 a0 = m.frag[(SG_FRAG_F2 - SG_FRAG_F1 + mt * 2 + 0) * 64 + lane];
+        const bf16x8 a1 = m.frag[(SG_FRAG_F2 - SG_FRAG_F1 + mt * 2 + 1) * 64 + lane];
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             f32x4 c = mfma(a0, H[0][nt], zero);
             c = mfma(a1, H[1][nt], c);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const float z = c[i] + m.p[SG_P_B2 + mt * 16 + g * 4 + i];
-                acc[nt] = fmaf(m.p[SG_P_W3 + mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc[nt]);
+                const float z = c[i] + m.b2[mt * 16 + g * 4 + i];
+                acc[nt] = fmaf(m.w3[mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc[nt]);
             }
         }
     }
@@ -279,7 +311,7 @@ __device__ __forceinline__ void mlp_infer(const Mlp &m, const bf16x8 E[4], float
         float v = acc[nt];
         v += __shfl_xor(v, 16, 64);
         v += __shfl_xor(v, 32, 64);
-        y[nt] = v + m.p[SG_P_B3];
+        y[nt] = v + *m.b3;
     }
 }
 
@@ -294,7 +326,7 @@ __device__ __forceinline__ float own(const float v[4])
  * brick b) as the renderer evaluates it: encoding from the half-precision
  * tables, rounded to bf16 for the first layer, MLP on MFMA.  Every lane of
  * the wavefront must call it (lanes without a sample pass any valid point). */
-__device__ __forceinline__ float sg_infer(const Enc &enc, const Mlp &mlp, const float x[3], int b)
+__device__ __forceinline__ float sg_infer(const Enc &enc, const MlpLds &mlp, const float x[3], int b)
 {
     SampleSet ss;
     gather_samples(x, b, ss);
@@ -356,9 +388,11 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
     __bf16 *sDZ2 = lds[wave], *sH1 = sDZ2 + 64 * SG_LDS_STRIDE, *sDZ1 = sH1 + 64 * SG_LDS_STRIDE,
            *sE = sDZ1 + 64 * SG_LDS_STRIDE;
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    f32x4 gW2[4][4], gW1[4][2];
+    /* weight-gradient tiles: wavefront w keeps the output row tiles 2w and 2w+1 of
+     * both products, summed over the samples of both wavefronts */
+    f32x4 gW2[2][4], gW1[2][2];
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
+    for (int a = 0; a < 2; a++) {
 #pragma unroll
         for (int b = 0; b < 4; b++) gW2[a][b] = zero;
         gW1[a][0] = zero;
@@ -405,7 +439,7 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
          * does (the fp32 tables are the optimiser's master copy): half the gather bytes,
          * and the network trained is the one rendered */
         bf16x8 E[4];
-        encode_frags(A.enc, ss, E);
+        encode_frags<true>(A.enc, ss, E);
         Fwd f;
         mlp_forward(A.mlp, E, f);
         /* loss gradient for the samples nt*16+l%16 */
@@ -468,21 +502,27 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
         __syncthreads();
         /* weight gradients, products over the 64 samples of the tile */
 #pragma unroll
-        for (int ks = 0; ks < 2; ks++) {
-            const int kc = ks * 32 + g * 8;
+        for (int src = 0; src < 2; src++) {
+            const __bf16 *tDZ2 = lds[src], *tH1 = tDZ2 + 64 * SG_LDS_STRIDE, *tDZ1 = tH1 + 64 * SG_LDS_STRIDE,
+                         *tE = tDZ1 + 64 * SG_LDS_STRIDE;
 #pragma unroll
-            for (int mo = 0; mo < 4; mo++) {
-                const bf16x8 a2 = *(const bf16x8 *)&sDZ2[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
-                const bf16x8 a1 = *(const bf16x8 *)&sDZ1[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+            for (int ks = 0; ks < 2; ks++) {
+                const int kc = ks * 32 + g * 8;
 #pragma unroll
-                for (int ni = 0; ni < 4; ni++) {
-                    const bf16x8 bh = *(const bf16x8 *)&sH1[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
-                    gW2[mo][ni] = mfma(a2, bh, gW2[mo][ni]);
-                }
+                for (int mm = 0; mm < 2; mm++) {
+                    const int mo = 2 * wave + mm;
+                    const bf16x8 a2 = *(const bf16x8 *)&tDZ2[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                    const bf16x8 a1 = *(const bf16x8 *)&tDZ1[(mo * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
 #pragma unroll
-                for (int ni = 0; ni < 2; ni++) {
-                    const bf16x8 be = *(const bf16x8 *)&sE[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
-                    gW1[mo][ni] = mfma(a1, be, gW1[mo][ni]);
+                    for (int ni = 0; ni < 4; ni++) {
+                        const bf16x8 bh = *(const bf16x8 *)&tH1[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                        gW2[mm][ni] = mfma(a2, bh, gW2[mm][ni]);
+                    }
+#pragma unroll
+                    for (int ni = 0; ni < 2; ni++) {
+                        const bf16x8 be = *(const bf16x8 *)&tE[(ni * 16 + (lane & 15)) * SG_LDS_STRIDE + kc];
+                        gW1[mm][ni] = mfma(a1, be, gW1[mm][ni]);
+                    }
                 }
             }
         }
@@ -542,16 +582,16 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
     }
     /* flush: weight tiles are unique per lane; per-output sums reduce over l%16 */
 #pragma unroll
-    for (int mo = 0; mo < 4; mo++)
+    for (int mm = 0; mm < 2; mm++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int o = mo * 16 + g * 4 + i;
+            const int o = (2 * wave + mm) * 16 + g * 4 + i;
 #pragma unroll
             for (int ni = 0; ni < 4; ni++)
-                unsafeAtomicAdd(&A.gparam[SG_P_W2 + o * SG_H + ni * 16 + (lane & 15)], gW2[mo][ni][i]);
+                unsafeAtomicAdd(&A.gparam[SG_P_W2 + o * SG_H + ni * 16 + (lane & 15)], gW2[mm][ni][i]);
 #pragma unroll
             for (int ni = 0; ni < 2; ni++)
-                unsafeAtomicAdd(&A.gparam[SG_P_W1 + o * SG_IN + ni * 16 + (lane & 15)], gW1[mo][ni][i]);
+                unsafeAtomicAdd(&A.gparam[SG_P_W1 + o * SG_IN + ni * 16 + (lane & 15)], gW1[mm][ni][i]);
         }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -974,7 +1014,8 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     unsigned long long *s_boff = (unsigned long long *)(smem + a.lutsize);
     int i, j;
     if (!block_pixel(a, blk, i, j)) return; /* uniform per workgroup */
-    stage(a, s_lut, s_boff);
+    const MlpLds mlp = stage_mlp(S.mlp, smem + a.lutsize + (a.nbricks + 1) / 2);
+    stage(a, s_lut, s_boff); /* its barrier covers the MLP staging too */
     const bool inside = (i < a.H) && (j < a.W);
     float d[3] = {0.0f, 0.0f, 1.0f};
     float tin = 0.0f, dt = 1.0f;
@@ -1040,7 +1081,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         const bool net = live && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
         if (!net) x[0] = x[1] = x[2] = 0.5f;
         float yv = 0.0f;
-        if (__ballot(net) != 0ull) yv = sg_infer(S.enc, S.mlp, x, b);
+        if (__ballot(net) != 0ull) yv = sg_infer(S.enc, mlp, x, b);
         if (live) {
             if (S.only_brick < 0 || S.only_brick == b) {
                 float s = yv;
@@ -1100,6 +1141,9 @@ __global__ __launch_bounds__(IVR_FRAME_BLOCK) void k_render_sg_items(const SgRen
 __global__ __launch_bounds__(256) void k_sg_points(const Enc enc, const Mlp mlp, int n, const float *__restrict__ xs,
                                                    const int *__restrict__ bs, float *__restrict__ ys)
 {
+    __shared__ __attribute__((aligned(16))) unsigned char s_mlp[SG_INFER_LDS];
+    const MlpLds m = stage_mlp(mlp, s_mlp);
+    __syncthreads();
     const int t = blockIdx.x * 256 + threadIdx.x;
     float x[3] = {0.5f, 0.5f, 0.5f};
     int b = 0;
@@ -1109,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_sg_points(const Enc enc, const Mlp mlp,
         x[2] = xs[3 * t + 2];
         b = bs[t];
     }
-    const float y = sg_infer(enc, mlp, x, b);
+    const float y = sg_infer(enc, m, x, b);
     if (t < n) ys[t] = y;
 }
 
@@ -1179,7 +1223,11 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
         int rc = ivr_launch_cells_on(st, &geo_cells, S.a, cells);
         if (rc) return rc;
     }
-    lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)S.a.nbricks * 8;
+    /* LDS: the LUT, the brick table (rounded to 16 bytes), the inference weights */
+    lds = (size_t)S.a.lutsize * sizeof(float4) + (size_t)((S.a.nbricks + 1) / 2) * 16 + SG_INFER_LDS;
+    if (lds > 65536)
+        return ivr_fail(IVR_ERR_UNSUPPORTED, "LUT of %d entries, %d bricks and the network exceed 64 KiB of LDS",
+                        S.a.lutsize, S.a.nbricks);
     S.skip = skip ? 1 : 0;
     S.nblocks = S.a.nblk_pad;
     return IVR_OK;
