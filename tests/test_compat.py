@@ -216,3 +216,30 @@ def test_cut_frames_through_dropin(refgpu):
             assert rc == 0
             assert np.array_equal(out.view(np.uint32), z["images"][k].view(np.uint32)), (use_gpu_frame, k)
     refgpu.ivrref_frame_on_gpu(0)
+
+
+@pytest.mark.gpu
+def test_compositetiles_through_dropin(refgpu):
+    """ivr_compat_compositetiles (compositetiles, main.c:1203-1218, on the GPU)
+    gives the compiled reference's output bit for bit: on the golden tiles (5
+    premultiplied 16x16 tiles over a zero buffer) and on ragged 37x23 tiles
+    over a non-zero buffer against the compiled reference run on the same
+    inputs"""
+    import oracle
+    z = np.load(os.path.join(G, "composite.npz"))
+    fn = refgpu.ivr_compat_compositetiles  # libivr_compat.so, loaded by the relinked host
+    dst = np.zeros((16, 16, 4), np.float32)
+    assert fn(f(dst), f(np.ascontiguousarray(z["tiles"])), 5, 16, 16) == 0
+    assert np.array_equal(dst.view(np.uint32), z["out"].view(np.uint32))
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref/libivrref.so not built")
+    rng = np.random.default_rng(12)
+    a = rng.uniform(0.0, 1.0, (3, 23, 37, 1)).astype(np.float32)
+    tiles = np.ascontiguousarray(np.concatenate([(rng.uniform(0.0, 1.0, (3, 23, 37, 3)) * a).astype(np.float32), a], -1))
+    base = np.ascontiguousarray(rng.uniform(0.0, 0.5, (23, 37, 4)).astype(np.float32))
+    want = base.copy()
+    ref.ivrref_compositetiles(f(want), f(tiles.copy()), 3, 37, 23)
+    got = base.copy()
+    assert fn(f(got), f(tiles), 3, 37, 23) == 0
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

@@ -383,7 +383,9 @@ def test_minimum_sizes(ctx):
 
 def test_c2_full_size_rows_match_oracle(ctx):
     """C2 at its full size: 512^3 generated on the GPU, 1024^2 frame; a band
-    of rows checked against the oracle, determinism of repeated renders"""
+    of rows checked against the oracle (sort-last: per-brick tiles), every 8th
+    row of the fused and the sort-last frame against the one-pass oracle,
+    determinism of repeated renders"""
     cfg = configs.WORKLOADS[2]
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(2))
@@ -413,6 +415,14 @@ def test_c2_full_size_rows_match_oracle(ctx):
     ref32, _ = _oracle_sort_last(host, st, lut, cfg["brick"], rows=rows)
     m, rmse, same = _report("C2 sort-last rows %s vs oracle tiles" % (rows,), c8[sl], ref32[sl], c32[sl])
     assert same == 1.0
+    # every 8th row of both frames against the one-pass oracle
+    spread = list(range(0, 1024, 8))
+    for tag, img8, img32, p in (("fused", a8, a32, oracle.params()),
+                                ("sort-last", c8, c32, oracle.params(brick=cfg["brick"], sort_last=True))):
+        ref = np.stack([oracle.render(host, oracle.view_of(st), lut, p, rows=(r, r + 1))[0][r] for r in spread])
+        m, rmse, same = _report("C2 %s, %d rows spread over the frame" % (tag, len(spread)), img8[spread], ref,
+                                img32[spread])
+        assert same == 1.0
     print("C2 samples per frame: fused %d, sort-last %d" % (s1["samples"], s3["samples"]), flush=True)
     vol.close()
 
@@ -421,7 +431,8 @@ def test_c3_full_size_rows_match_oracle(ctx):
     """C3 at its full size: 1024^3 fp16 generated on the GPU (4x4x4 bricks,
     70 % zero), 1920x1080 with macrocell skipping; a band of rows of the
     sort-last and the fused frame bit-exact against the oracle (per-brick
-    tiles composited, and one pass), sample counts included"""
+    tiles composited, and one pass), and every 5th row of the frame against
+    the one-pass oracle"""
     cfg = configs.WORKLOADS[3]
     vol = ctx.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
     vol.generate(configs.gen_params(3))
@@ -443,6 +454,14 @@ def test_c3_full_size_rows_match_oracle(ctx):
         m, rmse, same = _report("C3 rows %s composite %d" % (rows, comp), img8[sl], ref32[sl], img32[sl])
         assert same == 1.0
         assert (img8[sl][..., 3] > 0).mean() > 0.05
+        # every 5th row of the whole frame against the one-pass oracle (sort-last: each
+        # ray's per-brick runs combined far to near, equal to the tiles by the oracle's own test)
+        p = oracle.params(brick=cfg["brick"], sort_last=True) if comp == ivr.SORT_LAST else oracle.params()
+        spread = list(range(0, H, 5))
+        ref = np.stack([oracle.render(host, oracle.view_of(st), lut, p, rows=(r, r + 1))[0][r] for r in spread])
+        m, rmse, same = _report("C3 %d rows spread over the frame, composite %d" % (len(spread), comp),
+                                img8[spread], ref, img32[spread])
+        assert same == 1.0
     vol.close()
 
 
