@@ -493,6 +493,7 @@ def surrogate_measure(ctx, vol, c, train_seconds, steps, cpu_seconds=0.0, keep=F
         "brick_psnr_db": [round(float(x), 2) for x in bpsnr],
         "no_fallback": {"ms_per_frame": round(sg_ms, 3), "psnr_db": round(p_sg, 2), "ssim": round(s_sg, 4)},
         "network_samples_per_frame": int(sg_stats["visited"]),
+        "network_lane_utilisation": round(sg_stats["visited"] / max(1, sg_stats["evaluations"]), 4),
         "plain_ms_per_frame": round(plain_ms, 3), "plain_fps": round(1000.0 / plain_ms, 2),
         "train": {"seconds": round(t_train, 1), "steps": nsteps, "samples_per_step": batch,
                   "ms_per_step": round(step_ms, 2),

@@ -114,6 +114,10 @@ typedef struct ivr_frame_stats {
                                    the macrocell pass (events)                  */
     float composite_ms;         /* device time of its sort-last combine pass    */
     unsigned long long partials; /* IVR_SORT_LAST: per-run partials written to HBM */
+    unsigned long long evaluations; /* surrogate frames: network lane slots evaluated
+                                       (64 per wavefront step that ran the network,
+                                       idle lanes included); visited / evaluations is
+                                       the network's lane utilisation              */
 } ivr_frame_stats;
 
 /* ---- context ---- */

@@ -50,7 +50,8 @@ class FrameDesc(C.Structure):
 
 class FrameStats(C.Structure):
     _fields_ = [("samples", C.c_ulonglong), ("visited", C.c_ulonglong),
-                ("render_ms", C.c_float), ("composite_ms", C.c_float), ("partials", C.c_ulonglong)]
+                ("render_ms", C.c_float), ("composite_ms", C.c_float), ("partials", C.c_ulonglong),
+                ("evaluations", C.c_ulonglong)]
 
 
 class SurrogateParams(C.Structure):
@@ -412,7 +413,7 @@ class Frame:
         st = FrameStats()
         check(lib.ivr_frame_stats_get(self.ctx.h, self.h, C.byref(st)))
         return dict(samples=st.samples, visited=st.visited, render_ms=st.render_ms,
-                    composite_ms=st.composite_ms, partials=st.partials)
+                    composite_ms=st.composite_ms, partials=st.partials, evaluations=st.evaluations)
 
     def close(self):
         if self.h:

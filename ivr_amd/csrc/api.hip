@@ -268,6 +268,7 @@ int ivr_frame_stats_get(ivr_ctx *ctx, ivr_frame *f, ivr_frame_stats *st)
     st->samples = sum[0];
     st->visited = sum[1];
     st->partials = sum[2];
+    st->evaluations = sum[3];
     st->render_ms = 0.0f;
     st->composite_ms = 0.0f;
     if (f->timed) {

@@ -24,7 +24,7 @@
 #endif
 #define IVR_SL_LDS (64 * IVR_FRAME_BLOCK) /* LDS for sort-last partials: 4 runs */
 /* frame counters: IVR_CTR_SLOTS stripes of IVR_CTRS u64 (samples, visited,
- * partials, spare), a workgroup adding to stripe blockIdx % IVR_CTR_SLOTS;
+ * partials, network lane slots evaluated), a workgroup adding to stripe blockIdx % IVR_CTR_SLOTS;
  * one address taking every wavefront's atomic was measured to slow a frame
  * by up to 40 % */
 #define IVR_CTR_SLOTS 256

@@ -28,6 +28,11 @@
 #define SG_FRAG_G1 20 /* W1^T, 2 x 2, permuted                 */
 #define SG_NFRAG 24
 
+/* bytes allocated past the half-precision tables: the encoding reads every
+ * x-neighbour pair of corners with one 8-byte load, which at the last entry
+ * of the last table reads 4 bytes beyond it (and discards them) */
+#define SG_T16_TAIL 16
+
 struct ivr_surrogate {
     ivr_surrogate_params sp;
     unsigned T = 0;

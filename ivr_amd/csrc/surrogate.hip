@@ -89,15 +89,21 @@ __device__ __forceinline__ Lvl lvl_of(const Enc &e, int g, int a, int step)
 __device__ __forceinline__ unsigned corner_index(unsigned T, const Lvl &L, int x, int y, int z)
 {
     const unsigned n1 = L.n1;
-    if (n1) return (unsigned)x + n1 * ((unsigned)y + n1 * (unsigned)z);
+    /* both forms computed and one selected: the lane groups of a wavefront
+     * work on levels of both kinds, and a branch here would split the
+     * gathers into separately scheduled blocks */
+    const unsigned dense = (unsigned)x + n1 * ((unsigned)y + n1 * (unsigned)z);
     /* hashed levels: the 4x4x4 block of corners hashes to a slot of 64
      * entries (the spatial hash of Mueller et al. 2022 on block coordinates)
-     * holding its corners in Morton order, so a cell's 8 corners take about
-     * 2.3 cache lines instead of about 4 */
+     * holding its corners at x1 x0 | y0 | z0 | y1 | z1 (low bits first): rows
+     * of 4 along x are contiguous, so a cell's x-neighbour corners are one
+     * 8-byte load unless x is the block's last, and a 64-byte group is a
+     * 4x2x2 brick of corners, so a cell's 8 corners take about 2.3 cache
+     * lines instead of about 4 */
     const unsigned bx = (unsigned)x >> 2, by = (unsigned)y >> 2, bz = (unsigned)z >> 2;
     const unsigned blk = (bx * 1u ^ by * 2654435761u ^ bz * 805459861u) & ((T >> 6) - 1u);
-    const unsigned local = (x & 1) | (y & 1) << 1 | (z & 1) << 2 | (x & 2) << 2 | (y & 2) << 3 | (z & 2) << 4;
-    return blk << 6 | local;
+    const unsigned local = (x & 3) | (y & 1) << 2 | (z & 1) << 3 | (y & 2) << 3 | (z & 2) << 4;
+    return n1 ? dense : (blk << 6 | local);
 }
 
 /* corner 0 and fractional weights of level L at local position x in [0,1] */
@@ -125,14 +131,35 @@ __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lv
     level_cell(L, x, v0, w);
     const size_t base = ((size_t)brick * SG_L + L.l) * (size_t)e.T;
     const __half2 *tb = e.table16 + base;
+    /* the corners (x, y, z) and (x+1, y, z) are consecutive entries on dense
+     * levels (x < N) and on hashed levels unless x is the last of its block:
+     * one 8-byte load per x-pair, and for that case a second load of the
+     * (x+1) corner into registers of its own (the half-precision copy has a
+     * 16-byte tail, so the pair load of the very last entry stays inside the
+     * allocation) */
+    const bool split = L.n1 == 0u && (v0[0] & 3) == 3;
+    unsigned pr[4][2], ex[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+        __builtin_memcpy(pr[r], tb + corner_index(e.T, L, v0[0], v0[1] + (r & 1), v0[2] + (r >> 1)), 8);
+    if (split) {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            __builtin_memcpy(&ex[r], tb + corner_index(e.T, L, v0[0] + 1, v0[1] + (r & 1), v0[2] + (r >> 1)), 4);
+    }
+    /* corners in the order c = dx + 2 dy + 4 dz, weights multiplied x, y, z */
     float2 f = make_float2(0.0f, 0.0f);
 #pragma unroll
-    for (int c = 0; c < 8; c++) {
-        const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
-        const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-        const float2 t = __half22float2(tb[corner_index(e.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz)]);
-        f.x = fmaf(wc, t.x, f.x);
-        f.y = fmaf(wc, t.y, f.y);
+    for (int r = 0; r < 4; r++) {
+        const int dy = r & 1, dz = r >> 1;
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+            const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
+            const unsigned u = dx ? (split ? ex[r] : pr[r][1]) : pr[r][0];
+            const float2 t = __half22float2(__builtin_bit_cast(__half2, u));
+            f.x = fmaf(wc, t.x, f.x);
+            f.y = fmaf(wc, t.y, f.y);
+        }
     }
     return f;
 }
@@ -267,52 +294,41 @@ __device__ __forceinline__ void mlp_forward(const Mlp &m, const bf16x8 E[4], Fwd
     }
 }
 
-/* The forward pass of inference: y of mlp_forward with the same float
- * operations in the same order (layer 3's sums run over mt, then i, for
- * each nt), but each layer-2 output tile is folded into y as soon as it is
- * made instead of being kept for a backward pass, which leaves 64 fewer
- * registers live across the product. */
-__device__ __forceinline__ void mlp_infer(const MlpLds &m, const bf16x8 E[4], float y[4])
+/* The forward pass of inference for one column of 16 samples (the B
+ * fragment E of samples nt*16 + l%16): y of mlp_forward for that column with
+ * the same float operations in the same order (layer 3's sum runs over mt,
+ * then i), each layer-2 output tile folded into y as soon as it is made. */
+__device__ __forceinline__ float mlp_infer_col(const MlpLds &m, const bf16x8 &E)
 {
     const int lane = threadIdx.x & 63, g = lane >> 4;
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    bf16x8 H[2][4];
+    bf16x8 H[2];
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
         const bf16x8 a = m.frag[mt * 64 + lane]; /* the staged sets start at SG_FRAG_F1 */
+        const f32x4 c = mfma(a, E, zero);
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            f32x4 c = mfma(a, E[nt], zero);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float z = c[i] + m.b1[mt * 16 + g * 4 + i];
-                H[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
-            }
+        for (int i = 0; i < 4; i++) {
+            const float z = c[i] + m.b1[mt * 16 + g * 4 + i];
+            H[mt >> 1][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
         }
     }
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float acc = 0.0f;
 #pragma unroll
     for (int mt = 0; mt < 4; mt++) {
         const bf16x8 a0 = m.frag[(SG_FRAG_F2 - SG_FRAG_F1 + mt * 2 + 0) * 64 + lane];
         const bf16x8 a1 = m.frag[(SG_FRAG_F2 - SG_FRAG_F1 + mt * 2 + 1) * 64 + lane];
+        f32x4 c = mfma(a0, H[0], zero);
+        c = mfma(a1, H[1], c);
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++) {
-            f32x4 c = mfma(a0, H[0][nt], zero);
-            c = mfma(a1, H[1][nt], c);
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const float z = c[i] + m.b2[mt * 16 + g * 4 + i];
-                acc[nt] = fmaf(m.w3[mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc[nt]);
-            }
+        for (int i = 0; i < 4; i++) {
+            const float z = c[i] + m.b2[mt * 16 + g * 4 + i];
+            acc = fmaf(m.w3[mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc);
         }
     }
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        float v = acc[nt];
-        v += __shfl_xor(v, 16, 64);
-        v += __shfl_xor(v, 32, 64);
-        y[nt] = v + *m.b3;
-    }
+    acc += __shfl_xor(acc, 16, 64);
+    acc += __shfl_xor(acc, 32, 64);
+    return acc + *m.b3;
 }
 
 /* own sample's value from the y[nt] a lane holds (its sample is nt = l/16) */
@@ -322,19 +338,47 @@ __device__ __forceinline__ float own(const float v[4])
     return g == 0 ? v[0] : (g == 1 ? v[1] : (g == 2 ? v[2] : v[3]));
 }
 
-/* The network value at each lane's sample (local position x in [0,1]^3 of
- * brick b) as the renderer evaluates it: encoding from the half-precision
- * tables, rounded to bf16 for the first layer, MLP on MFMA.  Every lane of
- * the wavefront must call it (lanes without a sample pass any valid point). */
-__device__ __forceinline__ float sg_infer(const Enc &enc, const MlpLds &mlp, const float x[3], int b)
+/* the levels lane group l/16 encodes: lvl_of(e, l/16, q, 4) for q = 0..3 */
+__device__ __forceinline__ void lane_levels(const Enc &e, Lvl lv[4])
 {
-    SampleSet ss;
-    gather_samples(x, b, ss);
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) lv[q] = lvl_of(e, g, q, 4);
+}
+
+/* The network value at each lane's sample (local position x in [0,1]^3 of
+ * brick b; lv from lane_levels) as the renderer evaluates it: encoding from
+ * the half-precision tables, rounded to bf16 for the first layer, MLP on
+ * MFMA.  The gathers of all four columns of 16 samples are issued before any
+ * column's MLP: measured on C4 and the 1024^3 surrogate at 1080p, one
+ * wavefront per SIMD with every gather of the wavefront in flight (41.8 and
+ * 31.0 ms per frame) beats two wavefronts each with two columns in flight
+ * (48.7, 34.2 ms) and three wavefronts gathering one column at a time (53.9,
+ * 36.4 ms).  Every lane of the wavefront must call it (lanes without a sample
+ * pass any valid point). */
+__device__ __forceinline__ float sg_infer(const Enc &enc, const Lvl lv[4], const MlpLds &mlp, const float x[3], int b)
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     bf16x8 E[4];
-    encode_frags(enc, ss, E);
-    float y[4];
-    mlp_infer(mlp, E, y);
-    return own(y);
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const int src = nt * 16 + (lane & 15);
+        const float xs[3] = {__shfl(x[0], src, 64), __shfl(x[1], src, 64), __shfl(x[2], src, 64)};
+        const int bs = __shfl(b, src, 64);
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float2 f = encode_level(enc, bs, lv[q], xs);
+            E[nt][2 * q] = (__bf16)f.x;
+            E[nt][2 * q + 1] = (__bf16)f.y;
+        }
+    }
+    float yown = 0.0f;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        const float y = mlp_infer_col(mlp, E[nt]);
+        if (g == nt) yown = y; /* a lane's own sample is column l/16 */
+    }
+    return yown;
 }
 
 /* ---------------------------------------------------------------- training */
@@ -756,7 +800,7 @@ int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogat
     s->ncells = vol->ncells;
     int rc;
     if ((rc = ivr_dalloc(ctx, (void **)&s->tables, s->ntable * 4)) ||
-        (rc = ivr_dalloc(ctx, (void **)&s->tables16, s->ntable * 2)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->tables16, s->ntable * 2 + SG_T16_TAIL)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->gtables, s->ntable * 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->mt, s->ntable * 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->vt, s->ntable * 4)) ||
@@ -805,7 +849,7 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
     if (ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         ivr_dfree(ctx, s->tables, s->ntable * 4);
-        ivr_dfree(ctx, s->tables16, s->ntable * 2);
+        ivr_dfree(ctx, s->tables16, s->ntable * 2 + SG_T16_TAIL);
         ivr_dfree(ctx, s->gtables, s->ntable * 4);
         ivr_dfree(ctx, s->mt, s->ntable * 4);
         ivr_dfree(ctx, s->vt, s->ntable * 4);
@@ -1032,7 +1076,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         }
     }
     Acc acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    unsigned cnt = 0, fetched = 0;
+    unsigned cnt = 0, fetched = 0, evals = 0;
     bool live = n > 0;
     int k = 0;
     const float invB = 1.0f / (float)(1 << S.sbshift);
@@ -1044,6 +1088,8 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         rdt = 1.0f / dt;
     }
     int cc = -1, e = 0; /* macrocell of the transparency code held in e */
+    Lvl lv[4];
+    lane_levels(S.enc, lv);
     while (__ballot(live) != 0ull) {
         float x[3] = {0.5f, 0.5f, 0.5f};
         int b = 0;
@@ -1081,7 +1127,10 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         const bool net = live && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
         if (!net) x[0] = x[1] = x[2] = 0.5f;
         float yv = 0.0f;
-        if (__ballot(net) != 0ull) yv = sg_infer(S.enc, mlp, x, b);
+        if (__ballot(net) != 0ull) {
+            yv = sg_infer(S.enc, lv, mlp, x, b);
+            evals++;
+        }
         if (live) {
             if (S.only_brick < 0 || S.only_brick == b) {
                 float s = yv;
@@ -1112,6 +1161,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     }
     count_add(a.counters, 0, cnt);
     count_add(a.counters, 1, fetched);
+    count_add(a.counters, 3, evals);
 }
 
 template <bool SKIP>
@@ -1153,7 +1203,9 @@ __global__ __launch_bounds__(256) void k_sg_points(const Enc enc, const Mlp mlp,
         x[2] = xs[3 * t + 2];
         b = bs[t];
     }
-    const float y = sg_infer(enc, m, x, b);
+    Lvl lv[4];
+    lane_levels(enc, lv);
+    const float y = sg_infer(enc, lv, m, x, b);
     if (t < n) ys[t] = y;
 }
 
@@ -1424,17 +1476,49 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
     return ivr_surrogate_set_fallback(ctx, s, flags.data());
 }
 
-/* checkpoint: "IVRSG003", params struct, dims, step, then MLP (params, m, v),
+/* checkpoint: "IVRSG004", params struct, dims, step, then MLP (params, m, v),
  * tables (values, m, v) and the macrocell ranges as little-endian float32.
- * "IVRSG002" files (a params struct without log2_group, which is then 0) and
- * "IVRSG001" files (without the ranges too) still load. */
+ * Older files still load: "IVRSG003" and before stored a hashed level's
+ * 4x4x4 block Morton-ordered in its slot (x0 y0 z0 x1 y1 z1), which is
+ * permuted to the current order on load; "IVRSG002" files have a params
+ * struct without log2_group (then 0), "IVRSG001" files no ranges either. */
+namespace {
+
+/* a table section [brick][level][T][2] written with the slots of hashed
+ * levels in Morton order (place x0 y0 z0 x1 y1 z1 of a block's corner),
+ * rearranged in place to corner_index's order */
+void morton_slots_to_rows(const ivr_surrogate *s, float *t)
+{
+    unsigned to[64];
+    for (unsigned m = 0; m < 64; m++) {
+        const unsigned x = (m & 1) | (m >> 2 & 2), y = (m >> 1 & 1) | (m >> 3 & 2), z = (m >> 2 & 1) | (m >> 4 & 2);
+        to[m] = x | (y & 1) << 2 | (z & 1) << 3 | (y & 2) << 3 | (z & 2) << 4;
+    }
+    float slot[128];
+    for (int b = 0; b < s->nbricks; b++)
+        for (int l = 0; l < SG_L; l++) {
+            if (s->dense_n1[l]) continue;
+            float *lv = t + ((size_t)b * SG_L + l) * (size_t)s->T * SG_F;
+            for (size_t q = 0; q < s->T; q += 64) {
+                float *p = lv + q * SG_F;
+                for (unsigned m = 0; m < 64; m++) {
+                    slot[to[m] * 2] = p[m * 2];
+                    slot[to[m] * 2 + 1] = p[m * 2 + 1];
+                }
+                memcpy(p, slot, sizeof slot);
+            }
+        }
+}
+
+} // namespace
+
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
 {
     if (!ctx || !s || !path) return ivr_fail(IVR_ERR_ARG, "null argument");
     if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     FILE *f = fopen(path, "wb");
     if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
-    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '3'};
+    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '4'};
     int hdr[8] = {s->nbricks, s->B, s->n[0], s->n[1], s->n[2], s->step, (int)s->T, SG_NPARAM};
     bool ok = fwrite(magic, 8, 1, f) == 1 && fwrite(&s->sp, sizeof s->sp, 1, f) == 1 &&
               fwrite(hdr, sizeof hdr, 1, f) == 1;
@@ -1459,7 +1543,7 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     char magic[8];
     ivr_surrogate_params sp;
     int hdr[8];
-    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG00", 7) && magic[7] >= '1' && magic[7] <= '3';
+    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG00", 7) && magic[7] >= '1' && magic[7] <= '4';
     /* versions 1 and 2 stored the struct up to seed */
     const size_t sp_bytes = ok && magic[7] < '3' ? offsetof(ivr_surrogate_params, log2_group) : sizeof sp;
     memset(&sp, 0, sizeof sp);
@@ -1490,6 +1574,8 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     }
     fclose(f);
     if (!ok) return ivr_fail(IVR_ERR_IO, "reading %s failed", path);
+    if (magic[7] < '4') /* Morton-ordered slots of the hashed levels to the current order */
+        for (int k = 3; k < 6; k++) morton_slots_to_rows(s, buf[k].data());
     for (int k = 0; k < nsec; k++)
         IVR_HIP(hipMemcpy(dev[k], buf[k].data(), cnt[k] * 4, hipMemcpyHostToDevice));
     s->sp = sp;

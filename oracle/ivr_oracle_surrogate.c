@@ -10,8 +10,9 @@
  *     (N+1)^3 vertices fit in T entries is indexed densely, otherwise the
  *     4x4x4 block of vertices holding the vertex is placed by the spatial
  *     hash (bx*1 ^ by*2654435761 ^ bz*805459861) mod T/64 of its block
- *     coordinates into a slot of 64 entries, Morton-ordered inside (a
- *     locality-keeping variant of the paper's per-vertex hash); features of
+ *     coordinates into a slot of 64 entries, at place x1 x0 | y0 | z0 | y1 |
+ *     z1 (low bits first) inside it (a locality-keeping variant of the
+ *     paper's per-vertex hash); features of
  *     the 8 surrounding vertices are blended trilinearly;
  *   - MLP 32 -> 64 -> 64 -> 1 with ReLU hidden layers and a linear output,
  *     mean-squared-error loss, analytic backward pass;
@@ -55,16 +56,13 @@ uint32_t ivros_index(int l, int nmin, double growth, uint32_t T, int x, int y, i
     const uint32_t n1 = (uint32_t)ivros_level_res(l, nmin, growth) + 1u;
     if ((uint64_t)n1 * n1 * n1 <= T) return (uint32_t)x + n1 * ((uint32_t)y + n1 * (uint32_t)z);
     /* hashed: block (x/4, y/4, z/4) -> slot of 64 entries by the primes
-     * 1, 2654435761, 805459861; the corner's place in the slot is the
-     * interleave x0 y0 z0 x1 y1 z1 of its low coordinate bits */
+     * 1, 2654435761, 805459861; the corner's place in the slot is, from the
+     * lowest bit up, x mod 4 (two bits), then y0, z0, y1, z1 of its low
+     * coordinate bits */
     const uint32_t bx = (uint32_t)x / 4u, by = (uint32_t)y / 4u, bz = (uint32_t)z / 4u;
     const uint32_t slot = (bx * 1u ^ by * 2654435761u ^ bz * 805459861u) % (T / 64u);
-    uint32_t local = 0;
-    for (int bit = 0; bit < 2; bit++) {
-        local |= (((uint32_t)x >> bit) & 1u) << (3 * bit);
-        local |= (((uint32_t)y >> bit) & 1u) << (3 * bit + 1);
-        local |= (((uint32_t)z >> bit) & 1u) << (3 * bit + 2);
-    }
+    const uint32_t lx = (uint32_t)x % 4u, ly = (uint32_t)y % 4u, lz = (uint32_t)z % 4u;
+    const uint32_t local = lx + 4u * (ly % 2u) + 8u * (lz % 2u) + 16u * (ly / 2u) + 32u * (lz / 2u);
     return slot * 64u + local;
 }
 

@@ -38,18 +38,27 @@ def test_level_resolutions_and_dense_split():
 
 def test_block_hash_layout():
     """hashed levels: a 4x4x4 block of vertices occupies one 64-entry slot
-    chosen by the spatial hash of its block coordinates, Morton-ordered inside
-    (x0 y0 z0 x1 y1 z1), so a cell whose corner 0 has even coordinates keeps
-    its 8 corners in 8 consecutive entries"""
+    chosen by the spatial hash of its block coordinates, at place
+    x1 x0 | y0 | z0 | y1 | z1 (low bits first) inside it: rows of 4 along x
+    are contiguous (a cell's x-neighbour corners are adjacent entries unless
+    x is the block's last), and a cell whose corner 0 has even coordinates
+    keeps its 8 corners in one 16-entry group"""
     T = 1 << 19
     idx = lambda x, y, z: oracle.lib.ivros_index(4, 16, 1.5, T, x, y, z)  # noqa: E731
     slot = (1 ^ 2654435761 ^ 805459861) & (T // 64 - 1)
-    assert idx(5, 6, 7) == slot * 64 + 53  # x=01 y=10 z=11 -> 1 + 4 + 16 + 32
+    assert idx(5, 6, 7) == slot * 64 + 57  # x=1 (bits 0-1), y0=0, z0=1 (bit 3), y1=1 (bit 4), z1=1 (bit 5)
+    assert idx(4, 4, 4) == slot * 64
     block = sorted(idx(4 + a, 8 + b, 12 + c) for a in range(4) for b in range(4) for c in range(4))
     base = ((1 ^ 2 * 2654435761 ^ 3 * 805459861) & (T // 64 - 1)) * 64
     assert block == list(range(base, base + 64))
+    for x in range(4, 7):  # x-pairs inside a block are adjacent entries, in order
+        for y in range(8, 12):
+            for z in range(12, 16):
+                assert idx(x + 1, y, z) == idx(x, y, z) + 1
     cell = sorted(idx(6 + (c & 1), 10 + (c >> 1 & 1), 14 + (c >> 2)) for c in range(8))
-    assert cell == list(range(cell[0], cell[0] + 8)) and cell[0] % 8 == 0
+    assert cell[0] % 16 == 2 and cell[-1] - cell[0] < 16 and cell == sorted(set(cell))
+    # the dense levels keep x-pairs adjacent too: x + n1 (y + n1 z) with x < N
+    assert oracle.lib.ivros_index(2, 16, 1.5, T, 36, 5, 7) == oracle.lib.ivros_index(2, 16, 1.5, T, 35, 5, 7) + 1
 
 
 def test_mlp_and_encoding_gradients_match_finite_differences():
@@ -108,14 +117,14 @@ def test_adam_first_step_closed_form():
 def _corner_coords(T, l, n1, hashed):
     """per table entry of level l, the vertex coordinates it holds: dense
     entries decode x + n1 (y + n1 z); a hashed entry's place in its 64-entry
-    slot decodes the Morton interleave x0 y0 z0 x1 y1 z1 to the vertex's
+    slot decodes x1 x0 | y0 | z0 | y1 | z1 (low bits first) to the vertex's
     coordinates inside its 4x4x4 block"""
     e = np.arange(T, dtype=np.int64)
     if not hashed:
         return e % n1, (e // n1) % n1, e // (n1 * n1)
     loc = e & 63
     bit = lambda k: (loc >> k) & 1  # noqa: E731
-    return bit(0) | bit(3) << 1, bit(1) | bit(4) << 1, bit(2) | bit(5) << 1
+    return loc & 3, bit(2) | bit(4) << 1, bit(3) | bit(5) << 1
 
 
 def test_encoding_reproduces_affine_fields():
@@ -123,7 +132,7 @@ def test_encoding_reproduces_affine_fields():
     coordinates encodes every point as that function of x * N (dense levels:
     global coordinates; hashed levels: coordinates inside the 4x4x4 block,
     for cells inside one block, whatever slots the blocks share); a wrong
-    corner weight, axis or Morton bit breaks it"""
+    corner weight, axis or place bit breaks it"""
     T = 1 << 17  # levels 0-2 dense (17^3, 25^3, 37^3 entries), 3-15 hashed
     rng = np.random.default_rng(5)
     table = np.zeros((16, T, 2), np.float32)
@@ -308,6 +317,33 @@ def test_gpu_gradients_match_oracle(ctx, small_vol):
     sg.close()
 
 
+def _write_morton_checkpoint(src, dst, sg):
+    """rewrite checkpoint src (format 4) as format 3: the same sections with
+    every hashed level's slots in Morton order, the layout of that format"""
+    import ctypes
+    raw = bytearray(open(src, "rb").read())
+    assert raw[:8] == b"IVRSG004"
+    off = 8 + ctypes.sizeof(ivr.SurrogateParams) + 8 * 4
+    nparam = 6337
+    ntab = sg.nbricks * 16 * sg.T * 2
+    to = np.zeros(64, np.int64)  # Morton place -> place in the current order
+    for m in range(64):
+        x = (m & 1) | (m >> 2 & 2)
+        y = (m >> 1 & 1) | (m >> 3 & 2)
+        z = (m >> 2 & 1) | (m >> 4 & 2)
+        to[m] = x | (y & 1) << 2 | (z & 1) << 3 | (y & 2) << 3 | (z & 2) << 4
+    hashed = [(r + 1) ** 3 > sg.T for r in sg.res]
+    for k in range(3):  # table values, first and second moments
+        o = off + 4 * (3 * nparam + k * ntab)
+        t = np.frombuffer(raw, np.float32, ntab, o).reshape(sg.nbricks, 16, sg.T // 64, 64, 2).copy()
+        for lvl in range(16):
+            if hashed[lvl]:
+                t[:, lvl] = t[:, lvl][:, :, to, :]
+        raw[o:o + 4 * ntab] = t.tobytes()
+    raw[:8] = b"IVRSG003"
+    open(dst, "wb").write(bytes(raw))
+
+
 @pytest.mark.gpu
 def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
     vol, host = small_vol
@@ -356,6 +392,15 @@ def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
     sg2.render(vol, tf, d, fr)
     assert np.array_equal(fr.download(), before)
     sg2.close()
+    # a file of the previous format (hashed slots in Morton order, x0 y0 z0 x1 y1 z1)
+    # loads into the current order: the same render
+    old = os.path.join(os.path.dirname(path), "sg3.ckpt")
+    _write_morton_checkpoint(path, old, sg)
+    sg3 = ctx.surrogate(vol, log2_table=15, seed=7)
+    sg3.load(old)
+    sg3.render(vol, tf, d, fr)
+    assert np.array_equal(fr.download(), before)
+    sg3.close()
     sg.close()
     fr.close()
     tf.close()
