@@ -46,6 +46,8 @@ struct ivr_surrogate {
     size_t ntable = 0;
     float *tables = nullptr, *gtables = nullptr, *mt = nullptr, *vt = nullptr;
     void *tables16 = nullptr; /* half-precision copy of tables, read by rendering */
+    float *dE = nullptr;      /* training: encoding gradients of one chunk of samples */
+    size_t dE_n = 0;          /* samples dE holds */
     float *params = nullptr, *gparams = nullptr, *mp = nullptr, *vp = nullptr;
     void *frag = nullptr;
     float *d_loss = nullptr;

@@ -164,40 +164,37 @@ __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lv
     return f;
 }
 
-/* first-layer B fragments: lane l, sample nt*16+l%16, features 8g..8g+7 */
-struct SampleSet {
-    float x[4][3]; /* local positions of samples nt*16 + l%16 */
-    int b[4];      /* their bricks */
-};
+/* the levels lane group l/16 encodes: lvl_of(e, l/16, q, 4) for q = 0..3 */
+__device__ __forceinline__ void lane_levels(const Enc &e, Lvl lv[4])
+{
+    const int g = (threadIdx.x & 63) >> 4;
+#pragma unroll
+    for (int q = 0; q < 4; q++) lv[q] = lvl_of(e, g, q, 4);
+}
 
-__device__ __forceinline__ void gather_samples(const float *x, int b, SampleSet &s)
+/* The first layer's B fragments of the wavefront's 64 samples (lane l's
+ * own sample at local position x of brick b): E[nt] holds, for sample
+ * nt*16 + l%16, the features 8g..8g+7 (levels lv[0..3] of lane group g),
+ * rounded to bf16.  The gathers of all four columns are issued before any
+ * is blended: measured on C4 and the 1024^3 surrogate at 1080p, one
+ * wavefront per SIMD with every gather of the wavefront in flight (41.8 and
+ * 31.0 ms per frame) beats two wavefronts each with two columns in flight
+ * (48.7, 34.2 ms) and three wavefronts gathering one column at a time (53.9,
+ * 36.4 ms). */
+__device__ __forceinline__ void encode_cols(const Enc &enc, const Lvl lv[4], const float x[3], int b, bf16x8 E[4])
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
         const int src = nt * 16 + (lane & 15);
-        s.x[nt][0] = __shfl(x[0], src, 64);
-        s.x[nt][1] = __shfl(x[1], src, 64);
-        s.x[nt][2] = __shfl(x[2], src, 64);
-        s.b[nt] = __shfl(b, src, 64);
-    }
-}
-
-/* SERIAL: the gathers of one sample column at a time (a scheduling barrier
- * after each), which the register-bound training kernel needs */
-template <bool SERIAL = false>
-__device__ __forceinline__ void encode_frags(const Enc &e, const SampleSet &s, bf16x8 E[4])
-{
-    const int g = (threadIdx.x & 63) >> 4;
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
+        const float xs[3] = {__shfl(x[0], src, 64), __shfl(x[1], src, 64), __shfl(x[2], src, 64)};
+        const int bs = __shfl(b, src, 64);
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            const float2 f = encode_level(e, s.b[nt], lvl_of(e, g, q, 4), s.x[nt]);
+            const float2 f = encode_level(enc, bs, lv[q], xs);
             E[nt][2 * q] = (__bf16)f.x;
             E[nt][2 * q + 1] = (__bf16)f.y;
         }
-        if (SERIAL) __builtin_amdgcn_sched_barrier(0);
     }
 }
 
@@ -294,6 +291,54 @@ __device__ __forceinline__ void mlp_forward(const Mlp &m, const bf16x8 E[4], Fwd
     }
 }
 
+/* The training forward of 64 samples: the first layer's activations H
+ * (bf16 fragments, kept for the backward pass) and y for samples nt*16+l%16,
+ * with the float operations of mlp_forward; the second layer's
+ * pre-activations are folded into y tile by tile and recomputed by the
+ * backward pass (a few dozen MFMAs per tile) instead of being held in 64
+ * registers across it. */
+__device__ __forceinline__ void mlp_forward_h(const Mlp &m, const bf16x8 E[4], bf16x8 H[2][4], float y[4])
+{
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a = m.frag[(SG_FRAG_F1 + mt) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const f32x4 c = mfma(a, E[nt], zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = c[i] + m.p[SG_P_B1 + mt * 16 + g * 4 + i];
+                H[mt >> 1][nt][(mt & 1) * 4 + i] = (__bf16)(z > 0.0f ? z : 0.0f);
+            }
+        }
+    }
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int mt = 0; mt < 4; mt++) {
+        const bf16x8 a0 = m.frag[(SG_FRAG_F2 + mt * 2 + 0) * 64 + lane];
+        const bf16x8 a1 = m.frag[(SG_FRAG_F2 + mt * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            f32x4 c = mfma(a0, H[0][nt], zero);
+            c = mfma(a1, H[1][nt], c);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float z = c[i] + m.p[SG_P_B2 + mt * 16 + g * 4 + i];
+                acc[nt] = fmaf(m.p[SG_P_W3 + mt * 16 + g * 4 + i], z > 0.0f ? z : 0.0f, acc[nt]);
+            }
+        }
+    }
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        float v = acc[nt];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        y[nt] = v + m.p[SG_P_B3];
+    }
+}
+
 /* The forward pass of inference for one column of 16 samples (the B
  * fragment E of samples nt*16 + l%16): y of mlp_forward for that column with
  * the same float operations in the same order (layer 3's sum runs over mt,
@@ -338,40 +383,16 @@ __device__ __forceinline__ float own(const float v[4])
     return g == 0 ? v[0] : (g == 1 ? v[1] : (g == 2 ? v[2] : v[3]));
 }
 
-/* the levels lane group l/16 encodes: lvl_of(e, l/16, q, 4) for q = 0..3 */
-__device__ __forceinline__ void lane_levels(const Enc &e, Lvl lv[4])
-{
-    const int g = (threadIdx.x & 63) >> 4;
-#pragma unroll
-    for (int q = 0; q < 4; q++) lv[q] = lvl_of(e, g, q, 4);
-}
-
 /* The network value at each lane's sample (local position x in [0,1]^3 of
  * brick b; lv from lane_levels) as the renderer evaluates it: encoding from
  * the half-precision tables, rounded to bf16 for the first layer, MLP on
- * MFMA.  The gathers of all four columns of 16 samples are issued before any
- * column's MLP: measured on C4 and the 1024^3 surrogate at 1080p, one
- * wavefront per SIMD with every gather of the wavefront in flight (41.8 and
- * 31.0 ms per frame) beats two wavefronts each with two columns in flight
- * (48.7, 34.2 ms) and three wavefronts gathering one column at a time (53.9,
- * 36.4 ms).  Every lane of the wavefront must call it (lanes without a sample
+ * MFMA.  Every lane of the wavefront must call it (lanes without a sample
  * pass any valid point). */
 __device__ __forceinline__ float sg_infer(const Enc &enc, const Lvl lv[4], const MlpLds &mlp, const float x[3], int b)
 {
-    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const int g = (threadIdx.x & 63) >> 4;
     bf16x8 E[4];
-#pragma unroll
-    for (int nt = 0; nt < 4; nt++) {
-        const int src = nt * 16 + (lane & 15);
-        const float xs[3] = {__shfl(x[0], src, 64), __shfl(x[1], src, 64), __shfl(x[2], src, 64)};
-        const int bs = __shfl(b, src, 64);
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float2 f = encode_level(enc, bs, lv[q], xs);
-            E[nt][2 * q] = (__bf16)f.x;
-            E[nt][2 * q + 1] = (__bf16)f.y;
-        }
-    }
+    encode_cols(enc, lv, x, b, E);
     float yown = 0.0f;
 #pragma unroll
     for (int nt = 0; nt < 4; nt++) {
@@ -393,6 +414,8 @@ struct TrainArgs {
     float *gparam;        /* fp32 [SG_NPARAM] */
     float *loss;          /* sum of squared errors */
     long long nsamples;   /* samples of this step */
+    long long s0, nchunk; /* this launch: samples s0 .. s0 + nchunk - 1 of the step */
+    float *dE;            /* [nchunk][SG_IN] fp32: d(loss)/d(encoding) of the chunk's samples */
     float inv_n;
     unsigned seed, step;
     const float *xs;      /* explicit samples: local x,y,z [n][3] (or null) */
@@ -402,7 +425,6 @@ struct TrainArgs {
 };
 
 #define SG_LDS_STRIDE 72 /* bf16 per staged row: 64 samples + pad */
-#define SG_GSTRIDE 33    /* floats per sample row of the staged encoding gradient */
 
 /* Jarzynski & Olano, "Hash Functions for GPU Rendering" (JCGT 2020): pcg hash */
 __device__ __forceinline__ unsigned pcg_hash(unsigned v)
@@ -425,15 +447,79 @@ __device__ __forceinline__ float field_at(const RenderArgs &a, const unsigned lo
     return trilinear(v, gx - (float)j, gy - (float)i, gz - (float)k);
 }
 
-__global__ __launch_bounds__(128) void k_train(const TrainArgs A)
+/* p, with the compiler unable to see that it is the same value: address
+ * arithmetic on it cannot move ahead of this point.  Keeps a persistent
+ * kernel from computing its loop's (or its epilogue's) addresses once before
+ * the loop and holding them, spilled, across it. */
+template <typename T>
+__device__ __forceinline__ T *opaque(T *p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
+/* Sample sid of a training step: its surrogate brick b, local position x
+ * in [0,1]^3 and, with t, its target (the plain volume's trilinear value).
+ * Drawn samples go to the bricks in contiguous ranges of the step (brick
+ * floor(sid * nbricks / n)), so a chunk of the step trains one or two
+ * bricks, whose tables and gradients then stay in the caches.  Inside a
+ * brick the samples are stratified: with K whole tiles of 64 samples in the
+ * brick and m = floor(cbrt(K)), tile k < m^3 draws its samples uniformly in
+ * cell k of an m^3 grid over the brick (the rest uniformly in the brick), so
+ * each sample is uniform in its brick and the 64 samples of a wavefront lie
+ * close together (their table gradients at the coarse levels are summed in
+ * LDS by k_train_scatter).  Positions come from the pcg hash of (seed,
+ * step, sid).  Explicit samples are read from A.xs, A.bs, A.ts. */
+__device__ __forceinline__ void train_sample(const TrainArgs &A, long long sid, float x[3], int &b, float *t)
+{
+    if (A.xs) {
+        x[0] = A.xs[sid * 3];
+        x[1] = A.xs[sid * 3 + 1];
+        x[2] = A.xs[sid * 3 + 2];
+        b = A.bs[sid];
+        if (t) *t = A.ts[sid];
+        return;
+    }
+    const RenderArgs &va = A.va;
+    const long long nb = A.snbricks, n = A.nsamples;
+    b = (int)min(sid * nb / n, nb - 1);
+    const long long first = ((long long)b * n + nb - 1) / nb, last = ((long long)(b + 1) * n + nb - 1) / nb;
+    const long long k = (sid - first) >> 6, K = (last - first) >> 6;
+    int m = (int)cbrtf((float)K);
+    while ((long long)(m + 1) * (m + 1) * (m + 1) <= K) m++;
+    while (m > 0 && (long long)m * m * m > K) m--;
+    const bool strat = k < (long long)m * m * m;
+    const int cell[3] = {strat ? (int)(k % m) : 0, strat ? (int)((k / m) % m) : 0, strat ? (int)(k / ((long long)m * m)) : 0};
+    const float inv_m = strat ? 1.0f / (float)m : 1.0f;
+    const int bc[3] = {b % A.snb[0], (b / A.snb[0]) % A.snb[1], b / (A.snb[0] * A.snb[1])};
+    float gv[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        const unsigned h = pcg_hash(A.seed ^ pcg_hash(A.step * 3u + (unsigned)r) ^ pcg_hash((unsigned)sid * 3u + r));
+        const float v = (float)(h >> 8) * (1.0f / 16777216.0f);
+        const float u = fminf(((float)cell[r] + v) * inv_m, 0x1.fffffep-1f); /* in [0, 1) */
+        const int lo = bc[r] << A.sbshift;
+        const float ext = (float)min(1 << A.sbshift, (va.n[r] - 1) - lo);
+        gv[r] = (float)lo + u * ext;
+        x[r] = u * ext / (float)(1 << A.sbshift);
+    }
+    if (t) *t = field_at(va, va.boff, gv[0], gv[1], gv[2]);
+}
+
+/* The network part of a training step over one chunk of samples: encoding
+ * (all gathers of a wavefront in flight, as in rendering), MLP forward and
+ * backward on MFMA, the weight gradients (products over samples, operands
+ * staged in LDS), the loss, and d(loss)/d(encoding) of every sample written
+ * to A.dE for k_train_scatter.  A workgroup is two wavefronts of 64 samples;
+ * wavefront w keeps the output row tiles 2w and 2w+1 of both weight-gradient
+ * products, summed over the samples of both wavefronts. */
+__global__ __launch_bounds__(128) void k_train_net(const TrainArgs A)
 {
     __shared__ __attribute__((aligned(16))) __bf16 lds[2][64 * SG_LDS_STRIDE * 3 + 32 * SG_LDS_STRIDE];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
     __bf16 *sDZ2 = lds[wave], *sH1 = sDZ2 + 64 * SG_LDS_STRIDE, *sDZ1 = sH1 + 64 * SG_LDS_STRIDE,
            *sE = sDZ1 + 64 * SG_LDS_STRIDE;
     const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    /* weight-gradient tiles: wavefront w keeps the output row tiles 2w and 2w+1 of
-     * both products, summed over the samples of both wavefronts */
     f32x4 gW2[2][4], gW1[2][2];
 #pragma unroll
     for (int a = 0; a < 2; a++) {
@@ -445,47 +531,29 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
     float gb1[16], gb2[16], gw3[16], gb3 = 0.0f, lsum = 0.0f;
 #pragma unroll
     for (int q = 0; q < 16; q++) gb1[q] = gb2[q] = gw3[q] = 0.0f;
+    Lvl lv[4];
+    lane_levels(A.enc, lv);
 
-    const long long ntiles = (A.nsamples + 63) / 64;
-    const long long ngroups = (ntiles + 1) / 2;
+    const long long ngroups = ((A.nchunk + 63) / 64 + 1) / 2;
     for (long long grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {
-        const long long sid = (grp * 2 + wave) * 64 + lane;
-        const bool valid = sid < A.nsamples;
-        /* this lane's sample: brick, local position, target */
+        const Mlp mlp = {opaque(A.mlp.frag), opaque(A.mlp.p)};
+        const long long tile = (grp * 2 + wave) * 64; /* first sample of the wavefront, in the chunk */
+        const bool valid = tile + lane < A.nchunk;
         float x[3], t = 0.0f;
         int b = 0;
-        if (A.xs) {
-            const long long s = valid ? sid : 0;
-            x[0] = A.xs[s * 3];
-            x[1] = A.xs[s * 3 + 1];
-            x[2] = A.xs[s * 3 + 2];
-            b = A.bs[s];
-            t = A.ts[s];
-        } else {
-            const RenderArgs &va = A.va;
-            b = (int)(sid % A.snbricks);
-            const int bc[3] = {b % A.snb[0], (b / A.snb[0]) % A.snb[1], b / (A.snb[0] * A.snb[1])};
-            float gv[3];
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const unsigned h = pcg_hash(A.seed ^ pcg_hash(A.step * 3u + (unsigned)r) ^ pcg_hash((unsigned)sid * 3u + r));
-                const float u = (float)(h >> 8) * (1.0f / 16777216.0f);
-                const int lo = bc[r] << A.sbshift;
-                const float ext = (float)min(1 << A.sbshift, (va.n[r] - 1) - lo);
-                gv[r] = (float)lo + u * ext;
-                x[r] = u * ext / (float)(1 << A.sbshift);
-            }
-            t = field_at(va, va.boff, gv[0], gv[1], gv[2]);
-        }
-        SampleSet ss;
-        gather_samples(x, b, ss);
-        /* the forward pass reads the half-precision copy of the tables, as rendering
-         * does (the fp32 tables are the optimiser's master copy): half the gather bytes,
-         * and the network trained is the one rendered */
+        train_sample(A, A.s0 + (valid ? tile + lane : 0), x, b, &t);
         bf16x8 E[4];
-        encode_frags<true>(A.enc, ss, E);
-        Fwd f;
-        mlp_forward(A.mlp, E, f);
+        encode_cols(A.enc, lv, x, b, E);
+        /* encodings as ET[feature][sample] for the first layer's weight gradient,
+         * staged now so that E dies with the first layer (the previous tile's
+         * readers of this buffer finished before its last barrier) */
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) sE[(g * 8 + j) * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = E[nt][j];
+        bf16x8 H[2][4];
+        float yv[4];
+        mlp_forward_h(mlp, E, H, yv);
         /* loss gradient for the samples nt*16+l%16 */
         float dy[4];
 #pragma unroll
@@ -493,36 +561,43 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
             const int src = nt * 16 + (lane & 15);
             const float tn = __shfl(t, src, 64);
             const bool vn = __shfl((int)valid, src, 64) != 0;
-            const float r = f.y[nt] - tn;
+            const float r = yv[nt] - tn;
             dy[nt] = vn ? 2.0f * r * A.inv_n : 0.0f;
             if (g == 0 && vn) lsum = fmaf(r, r, lsum);
             if (g == 0) gb3 += dy[nt];
         }
-        if (A.ys && valid) A.ys[sid] = own(f.y);
-        /* output layer and second hidden layer: dZ2 = relu'(Z2) * W3 dy */
+        if (A.ys && valid) A.ys[A.s0 + tile + lane] = own(yv);
+        /* output layer and second hidden layer: dZ2 = relu'(Z2) * W3 dy, with each
+         * Z2 tile recomputed as the forward pass made it */
         bf16x8 DZ[2][4];
 #pragma unroll
-        for (int mt = 0; mt < 4; mt++)
+        for (int mt = 0; mt < 4; mt++) {
+            const bf16x8 a0 = mlp.frag[(SG_FRAG_F2 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = mlp.frag[(SG_FRAG_F2 + mt * 2 + 1) * 64 + lane];
 #pragma unroll
-            for (int nt = 0; nt < 4; nt++)
+            for (int nt = 0; nt < 4; nt++) {
+                f32x4 c = mfma(a0, H[0][nt], zero);
+                c = mfma(a1, H[1][nt], c);
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int o = mt * 16 + g * 4 + i;
-                    const float z = f.Z2[mt][nt][i];
+                    const float z = c[i] + mlp.p[SG_P_B2 + o];
                     gw3[mt * 4 + i] = fmaf(dy[nt], z > 0.0f ? z : 0.0f, gw3[mt * 4 + i]);
-                    const float dz = z > 0.0f ? A.mlp.p[SG_P_W3 + o] * dy[nt] : 0.0f;
+                    const float dz = z > 0.0f ? mlp.p[SG_P_W3 + o] * dy[nt] : 0.0f;
                     const __bf16 dzb = (__bf16)dz;
                     gb2[mt * 4 + i] += (float)dzb;
                     DZ[mt >> 1][nt][(mt & 1) * 4 + i] = dzb;
                     sDZ2[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = dzb;
-                    sH1[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = f.H[mt >> 1][nt][(mt & 1) * 4 + i];
+                    sH1[o * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = H[mt >> 1][nt][(mt & 1) * 4 + i];
                 }
+            }
+        }
         /* dH1T = W2^T dZ2T, masked by the first layer's ReLU */
         bf16x8 D1[2][4];
 #pragma unroll
         for (int mt = 0; mt < 4; mt++) {
-            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G2 + mt * 2 + 0) * 64 + lane];
-            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G2 + mt * 2 + 1) * 64 + lane];
+            const bf16x8 a0 = mlp.frag[(SG_FRAG_G2 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = mlp.frag[(SG_FRAG_G2 + mt * 2 + 1) * 64 + lane];
 #pragma unroll
             for (int nt = 0; nt < 4; nt++) {
                 f32x4 c = mfma(a0, DZ[0][nt], zero);
@@ -530,7 +605,7 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const int o = mt * 16 + g * 4 + i;
-                    const float h1 = (float)f.H[mt >> 1][nt][(mt & 1) * 4 + i];
+                    const float h1 = (float)H[mt >> 1][nt][(mt & 1) * 4 + i];
                     const __bf16 d = (__bf16)(h1 > 0.0f ? c[i] : 0.0f);
                     gb1[mt * 4 + i] += (float)d;
                     D1[mt >> 1][nt][(mt & 1) * 4 + i] = d;
@@ -538,13 +613,22 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
                 }
             }
         }
-        /* encodings as ET[feature][sample] for the first layer's weight gradient */
+        /* dET = W1^T dZ1T: the encoding gradient of samples nt*16+l%16, features
+         * mt*16+4g..+3, to the chunk buffer for the table scatter */
 #pragma unroll
-        for (int nt = 0; nt < 4; nt++)
+        for (int mt = 0; mt < 2; mt++) {
+            const bf16x8 a0 = mlp.frag[(SG_FRAG_G1 + mt * 2 + 0) * 64 + lane];
+            const bf16x8 a1 = mlp.frag[(SG_FRAG_G1 + mt * 2 + 1) * 64 + lane];
 #pragma unroll
-            for (int j = 0; j < 8; j++) sE[(g * 8 + j) * SG_LDS_STRIDE + nt * 16 + (lane & 15)] = E[nt][j];
+            for (int nt = 0; nt < 4; nt++) {
+                f32x4 c = mfma(a0, D1[0][nt], zero);
+                c = mfma(a1, D1[1][nt], c);
+                const long long sn = tile + nt * 16 + (lane & 15);
+                if (sn < A.nchunk) *(f32x4 *)&A.dE[sn * SG_IN + mt * 16 + g * 4] = c;
+            }
+        }
         __syncthreads();
-        /* weight gradients, products over the 64 samples of the tile */
+        /* weight gradients, products over the 64 samples of each wavefront's tile */
 #pragma unroll
         for (int src = 0; src < 2; src++) {
             const __bf16 *tDZ2 = lds[src], *tH1 = tDZ2 + 64 * SG_LDS_STRIDE, *tDZ1 = tH1 + 64 * SG_LDS_STRIDE,
@@ -571,60 +655,9 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
             }
         }
         __syncthreads();
-        /* dET = W1^T dZ1T into LDS (the weight-gradient operands are done with),
-         * with each sample's position and brick (-1: past the batch) */
-        float *sG = (float *)sDZ2;  /* [64 samples][SG_GSTRIDE] */
-        float *sX = (float *)sH1;   /* [64 samples][4]: x, y, z, brick */
-#pragma unroll
-        for (int mt = 0; mt < 2; mt++) {
-            const bf16x8 a0 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 0) * 64 + lane];
-            const bf16x8 a1 = A.mlp.frag[(SG_FRAG_G1 + mt * 2 + 1) * 64 + lane];
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) {
-                f32x4 c = mfma(a0, D1[0][nt], zero);
-                c = mfma(a1, D1[1][nt], c);
-#pragma unroll
-                for (int i = 0; i < 4; i++) sG[(nt * 16 + (lane & 15)) * SG_GSTRIDE + mt * 16 + g * 4 + i] = c[i];
-            }
-        }
-        if (g == 0) {
-#pragma unroll
-            for (int nt = 0; nt < 4; nt++) {
-                const int s = nt * 16 + lane;
-                const bool vn = s + ((grp * 2 + wave) * 64) < A.nsamples;
-                sX[s * 4 + 0] = ss.x[nt][0];
-                sX[s * 4 + 1] = ss.x[nt][1];
-                sX[s * 4 + 2] = ss.x[nt][2];
-                sX[s * 4 + 3] = __int_as_float(vn ? ss.b[nt] : -1);
-            }
-        }
-        __syncthreads();
-        /* the table gradient: 16 lanes per (sample, level), one lane per corner and
-         * feature, so the atomics of one instruction fall on the few cache lines
-         * of four cells instead of on 64 unrelated lines */
-        {
-            const int corner = lane & 7, feat = (lane >> 3) & 1;
-            const int dx = corner & 1, dy = (corner >> 1) & 1, dz = corner >> 2;
-#pragma unroll 4
-            for (int round = 0; round < 256; round++) {
-                const int s = round >> 2, a = (round & 3) * 4;
-                const int bb = __float_as_int(sX[s * 4 + 3]);
-                if (bb < 0) continue; /* uniform over the wavefront: one sample per round */
-                const float x[3] = {sX[s * 4 + 0], sX[s * 4 + 1], sX[s * 4 + 2]};
-                const Lvl L = lvl_of(A.enc, g, a, 1);
-                int v0[3];
-                float w[3];
-                level_cell(L, x, v0, w);
-                const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
-                const unsigned idx = corner_index(A.enc.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz);
-                const float gr = sG[s * SG_GSTRIDE + L.l * 2 + feat];
-                float *tb = A.gtable + ((size_t)bb * SG_L + L.l) * (size_t)A.enc.T * SG_F;
-                unsafeAtomicAdd(tb + (size_t)idx * 2 + feat, wc * gr);
-            }
-        }
-        __syncthreads();
     }
     /* flush: weight tiles are unique per lane; per-output sums reduce over l%16 */
+    float *const gp = opaque(A.gparam);
 #pragma unroll
     for (int mm = 0; mm < 2; mm++)
 #pragma unroll
@@ -632,10 +665,10 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
             const int o = (2 * wave + mm) * 16 + g * 4 + i;
 #pragma unroll
             for (int ni = 0; ni < 4; ni++)
-                unsafeAtomicAdd(&A.gparam[SG_P_W2 + o * SG_H + ni * 16 + (lane & 15)], gW2[mm][ni][i]);
+                unsafeAtomicAdd(&gp[SG_P_W2 + o * SG_H + ni * 16 + (lane & 15)], gW2[mm][ni][i]);
 #pragma unroll
             for (int ni = 0; ni < 2; ni++)
-                unsafeAtomicAdd(&A.gparam[SG_P_W1 + o * SG_IN + ni * 16 + (lane & 15)], gW1[mm][ni][i]);
+                unsafeAtomicAdd(&gp[SG_P_W1 + o * SG_IN + ni * 16 + (lane & 15)], gW1[mm][ni][i]);
         }
 #pragma unroll
     for (int q = 0; q < 16; q++) {
@@ -648,9 +681,9 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
         }
         if ((lane & 15) == 0) {
             const int o = (q >> 2) * 16 + g * 4 + (q & 3);
-            unsafeAtomicAdd(&A.gparam[SG_P_B1 + o], v1);
-            unsafeAtomicAdd(&A.gparam[SG_P_B2 + o], v2);
-            unsafeAtomicAdd(&A.gparam[SG_P_W3 + o], v3);
+            unsafeAtomicAdd(&gp[SG_P_B1 + o], v1);
+            unsafeAtomicAdd(&gp[SG_P_B2 + o], v2);
+            unsafeAtomicAdd(&gp[SG_P_W3 + o], v3);
         }
     }
 #pragma unroll
@@ -659,8 +692,154 @@ __global__ __launch_bounds__(128) void k_train(const TrainArgs A)
         lsum += __shfl_xor(lsum, off, 64);
     }
     if (lane == 0) {
-        unsafeAtomicAdd(&A.gparam[SG_P_B3], gb3);
+        unsafeAtomicAdd(&gp[SG_P_B3], gb3);
         unsafeAtomicAdd(A.loss, lsum);
+    }
+}
+
+/* The hash-table gradient of one chunk: d(loss)/d(encoding) of each sample
+ * (k_train_net's A.dE) added to the 8 corners of its cell at every level.
+ * Each wavefront takes 64 samples, staged in LDS with their positions; 16
+ * lanes work on one (sample, level), one lane per corner and feature, so
+ * the atomics of one instruction fall on the few cache lines of four cells
+ * instead of on 64 unrelated lines.  Levels whose vertices under the
+ * wavefront's samples form a box of at most SG_POOL vertices in all (the
+ * coarse levels when the samples lie close together, as drawn samples do)
+ * are summed in LDS first and added to the table once per touched vertex;
+ * the other levels go straight to the table with hardware fp32 atomics. */
+#define SG_SC_WAVES 4
+#define SG_POOL 1024 /* vertices (2 features each) of a wavefront's LDS sums */
+
+/* the box of level l's vertices under samples with local positions in
+ * [xmin, xmax]: first vertex lo, edge d per axis (level_cell's clamping) */
+__device__ __forceinline__ void level_box(const Lvl &L, const float xmin[3], const float xmax[3], int lo[3], int d[3])
+{
+    const float N = (float)L.res;
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        const int c0 = min(max((int)floorf(xmin[a] * N), 0), L.res - 1);
+        const int c1 = min(max((int)floorf(xmax[a] * N), 0), L.res - 1);
+        lo[a] = c0;
+        d[a] = c1 - c0 + 2;
+    }
+}
+
+__global__ __launch_bounds__(64 * SG_SC_WAVES) void k_train_scatter(const TrainArgs A)
+{
+    __shared__ float s_g[SG_SC_WAVES][64 * SG_IN];  /* [sample][feature] */
+    __shared__ float4 s_x[SG_SC_WAVES][64];         /* x, y, z, brick (-1: past the chunk) */
+    __shared__ __attribute__((aligned(16))) float s_pool[SG_SC_WAVES][2 * SG_POOL]; /* LDS sums of the pooled levels */
+    __shared__ int s_plan[SG_SC_WAVES][SG_L][4];    /* per level: first pool vertex (-1: not pooled), lo */
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, g = lane >> 4;
+    const int corner = lane & 7, feat = (lane >> 3) & 1;
+    const int dx = corner & 1, dy = (corner >> 1) & 1, dz = corner >> 2;
+    float *pool = s_pool[wave];
+    for (int k = lane; k < 2 * SG_POOL; k += 64) pool[k] = 0.0f;
+    /* lane group g works on level 4q + g in pass q */
+    Lvl lv[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) lv[q] = lvl_of(A.enc, g, 4 * q, 1);
+    /* every wavefront of a workgroup runs the same number of tiles (the loop
+     * has barriers): tiles past the chunk have no samples */
+    const long long ntiles = ((A.nchunk + 63) / 64 + SG_SC_WAVES - 1) / SG_SC_WAVES * SG_SC_WAVES;
+    for (long long tl = (long long)blockIdx.x * SG_SC_WAVES + wave; tl < ntiles; tl += (long long)gridDim.x * SG_SC_WAVES) {
+        const long long tile = tl * 64;
+        const bool valid = tile + lane < A.nchunk;
+        float x[3];
+        int b = 0;
+        train_sample(A, A.s0 + (valid ? tile + lane : 0), x, b, nullptr);
+        s_x[wave][lane] = make_float4(x[0], x[1], x[2], __int_as_float(valid ? b : -1));
+        const f32x4 *src = (const f32x4 *)(A.dE + tile * SG_IN);
+        f32x4 *dst = (f32x4 *)s_g[wave];
+#pragma unroll
+        for (int k = 0; k < 64 * SG_IN / 4 / 64; k++) {
+            const long long e = (long long)k * 64 + lane; /* f32x4 index in the tile */
+            if (tile + e / (SG_IN / 4) < A.nchunk) dst[e] = src[e];
+        }
+        /* the wavefront's position box and brick; its levels pooled in LDS in
+         * order while their vertex boxes fit */
+        float xmin[3], xmax[3]; /* over every lane: a lane past the chunk only widens the box */
+        int bmin = valid ? b : INT_MAX, bmax = valid ? b : -1;
+#pragma unroll
+        for (int a = 0; a < 3; a++) xmin[a] = xmax[a] = x[a];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                xmin[a] = fminf(xmin[a], __shfl_xor(xmin[a], off, 64));
+                xmax[a] = fmaxf(xmax[a], __shfl_xor(xmax[a], off, 64));
+            }
+            bmin = min(bmin, __shfl_xor(bmin, off, 64));
+            bmax = max(bmax, __shfl_xor(bmax, off, 64));
+        }
+        if (lane < SG_L) s_plan[wave][lane][0] = -1;
+        if (lane == 0 && bmin == bmax) {
+            int used = 0;
+            for (int l = 0; l < SG_L; l++) {
+                const Lvl L = {l, A.enc.res[l], A.enc.n1[l]};
+                int lo[3], d[3];
+                level_box(L, xmin, xmax, lo, d);
+                const int vol = d[0] * d[1] * d[2];
+                if (used + vol > SG_POOL) break;
+                s_plan[wave][l][0] = used;
+                s_plan[wave][l][1] = lo[0] | d[0] << 16;
+                s_plan[wave][l][2] = lo[1] | d[1] << 16;
+                s_plan[wave][l][3] = lo[2] | d[2] << 16;
+                used += vol;
+            }
+        }
+        __syncthreads();
+        for (int s = 0; s < 64; s++) {
+            const float4 xs = s_x[wave][s];
+            const int bb = __float_as_int(xs.w);
+            if (bb < 0) continue; /* uniform over the wavefront */
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const Lvl &L = lv[q];
+                const float xp[3] = {xs.x, xs.y, xs.z};
+                int v0[3];
+                float w[3];
+                level_cell(L, xp, v0, w);
+                const float wc = (dx ? w[0] : 1.0f - w[0]) * (dy ? w[1] : 1.0f - w[1]) * (dz ? w[2] : 1.0f - w[2]);
+                const float gr = s_g[wave][s * SG_IN + L.l * 2 + feat];
+                const int base = s_plan[wave][L.l][0];
+                if (base >= 0) {
+                    const int px = s_plan[wave][L.l][1], py = s_plan[wave][L.l][2], pz = s_plan[wave][L.l][3];
+                    const int e = base + (v0[0] + dx - (px & 0xffff)) +
+                                  (px >> 16) * ((v0[1] + dy - (py & 0xffff)) + (py >> 16) * (v0[2] + dz - (pz & 0xffff)));
+                    atomicAdd(&pool[2 * e + feat], wc * gr); /* LDS */
+                } else {
+                    const unsigned idx = corner_index(A.enc.T, L, v0[0] + dx, v0[1] + dy, v0[2] + dz);
+                    float *tb = A.gtable + ((size_t)bb * SG_L + L.l) * (size_t)A.enc.T * SG_F;
+                    unsafeAtomicAdd(tb + (size_t)idx * 2 + feat, wc * gr);
+                }
+            }
+        }
+        /* the pooled sums to the table, one atomic per touched vertex and
+         * feature, and the pool cleared for the next tile */
+        if (bmin == bmax && bmin >= 0) {
+            float *tb0 = A.gtable + (size_t)bmin * SG_L * (size_t)A.enc.T * SG_F;
+            for (int l = 0; l < SG_L; l++) {
+                const int base = s_plan[wave][l][0];
+                if (base < 0) break;
+                const int px = s_plan[wave][l][1], py = s_plan[wave][l][2], pz = s_plan[wave][l][3];
+                const int ex = px >> 16, ey = py >> 16, ez = pz >> 16;
+                const Lvl L = {l, A.enc.res[l], A.enc.n1[l]};
+                float *tb = tb0 + (size_t)l * A.enc.T * SG_F;
+                for (int k = lane; k < ex * ey * ez; k += 64) {
+                    const float2 v = *(const float2 *)&pool[2 * (base + k)];
+                    if (v.x != 0.0f || v.y != 0.0f) {
+                        const int vx = (px & 0xffff) + k % ex, vy = (py & 0xffff) + (k / ex) % ey,
+                                  vz = (pz & 0xffff) + k / (ex * ey);
+                        const unsigned idx = corner_index(A.enc.T, L, vx, vy, vz);
+                        unsafeAtomicAdd(tb + (size_t)idx * 2, v.x);
+                        unsafeAtomicAdd(tb + (size_t)idx * 2 + 1, v.y);
+                        *(float2 *)&pool[2 * (base + k)] = make_float2(0.0f, 0.0f);
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -862,6 +1041,7 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
         ivr_dfree(ctx, s->d_fallback, IVR_MAX_BRICKS);
         ivr_dfree(ctx, s->d_cellrange, sizeof(float2) * s->ncells);
         ivr_dfree(ctx, s->d_cellempty, s->ncells);
+        ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
     }
     delete s;
     return IVR_OK;
@@ -869,9 +1049,23 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
 
 namespace {
 
+/* samples per k_train_net / k_train_scatter launch pair: the chunk buffer of
+ * encoding gradients is SG_CHUNK x 128 bytes (256 MiB), allocated at the
+ * first training call */
+#define SG_CHUNK (1ll << 21)
+
 int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long long n, const float *xs, const int *bs,
                  const float *ts, float *ys)
 {
+    const long long chunk = std::min<long long>(SG_CHUNK, (n + 63) / 64 * 64);
+    if ((size_t)chunk > s->dE_n) {
+        ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
+        s->dE = nullptr;
+        s->dE_n = 0;
+        int rc = ivr_dalloc(ctx, (void **)&s->dE, (size_t)chunk * SG_IN * 4);
+        if (rc) return rc;
+        s->dE_n = (size_t)chunk;
+    }
     TrainArgs A;
     memset(&A, 0, sizeof A);
     if (vol) volume_args(A.va, vol);
@@ -885,6 +1079,7 @@ int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long lon
     A.gparam = s->gparams;
     A.loss = s->d_loss;
     A.nsamples = n;
+    A.dE = s->dE;
     A.inv_n = 1.0f / (float)n;
     A.seed = (unsigned)(s->sp.seed * 0x9e3779b1ull);
     A.step = (unsigned)s->step;
@@ -895,10 +1090,16 @@ int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long lon
     IVR_HIP(hipMemsetAsync(s->gtables, 0, s->ntable * 4, ctx->stream));
     IVR_HIP(hipMemsetAsync(s->gparams, 0, SG_NPARAM * 4, ctx->stream));
     IVR_HIP(hipMemsetAsync(s->d_loss, 0, 4, ctx->stream));
-    const long long groups = ((n + 63) / 64 + 1) / 2;
-    const unsigned grid = (unsigned)std::min<long long>(groups, 4096);
-    hipLaunchKernelGGL(k_train, dim3(grid), dim3(128), 0, ctx->stream, A);
-    IVR_HIP(hipGetLastError());
+    for (long long s0 = 0; s0 < n; s0 += chunk) {
+        A.s0 = s0;
+        A.nchunk = std::min(chunk, n - s0);
+        const long long groups = ((A.nchunk + 63) / 64 + 1) / 2;
+        hipLaunchKernelGGL(k_train_net, dim3((unsigned)std::min<long long>(groups, 2048)), dim3(128), 0, ctx->stream, A);
+        const long long tiles = (A.nchunk + 63) / 64;
+        hipLaunchKernelGGL(k_train_scatter, dim3((unsigned)std::min<long long>((tiles + SG_SC_WAVES - 1) / SG_SC_WAVES, 2048)),
+                           dim3(64 * SG_SC_WAVES), 0, ctx->stream, A);
+        IVR_HIP(hipGetLastError());
+    }
     return IVR_OK;
 }
 
@@ -1424,6 +1625,9 @@ int ivr_surrogate_freeze(ivr_ctx *ctx, ivr_surrogate *s)
         ivr_dfree(ctx, *p, SG_NPARAM * 4);
         *p = nullptr;
     }
+    ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
+    s->dE = nullptr;
+    s->dE_n = 0;
     s->frozen = true;
     return IVR_OK;
 }

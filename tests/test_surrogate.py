@@ -260,7 +260,13 @@ def small_vol(ctx):
 
 
 @pytest.mark.gpu
-def test_gpu_gradients_match_oracle(ctx, small_vol):
+@pytest.mark.parametrize("layout", ["scattered", "clustered"])
+def test_gpu_gradients_match_oracle(ctx, small_vol, layout):
+    """GPU gradients of explicit samples against the oracle.  Scattered:
+    uniform points of two bricks, every level's table gradient added by
+    direct atomics.  Clustered: each wavefront's 64 points in one brick and a
+    cube of edge 0.05, so the coarse levels' gradients are summed in LDS
+    before they reach the table (and the finer ones are not)."""
     vol, _ = small_vol
     sg = ctx.surrogate(vol, log2_table=12)
     rng = np.random.default_rng(3)
@@ -269,8 +275,14 @@ def test_gpu_gradients_match_oracle(ctx, small_vol):
     sg.set(mlp=p, table=tables[0], brick=0)
     sg.set(table=tables[1], brick=1)
     n = 200  # not a multiple of 64: exercises the padded tile
-    xyz = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
-    bricks = rng.integers(0, 2, n).astype(np.int32)
+    if layout == "scattered":
+        xyz = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+        bricks = rng.integers(0, 2, n).astype(np.int32)
+    else:
+        groups = (n + 63) // 64
+        corner = rng.uniform(0.0, 0.95, (groups, 3))
+        xyz = (np.repeat(corner, 64, 0)[:n] + rng.uniform(0.0, 0.05, (n, 3))).astype(np.float32)
+        bricks = np.repeat(np.arange(groups) % 2, 64)[:n].astype(np.int32)
     t = rng.uniform(0, 1, n).astype(np.float32)
     y, loss, g, gt0 = sg.grad(xyz, bricks, t, table_brick=0)
     # the training forward pass reads the half-precision copy of the tables
@@ -413,7 +425,8 @@ def test_grouped_bricks_and_freeze(ctx, small_vol):
     brick: the same training samples (equal first-step loss from equal
     initial weights) and, with the same weights, bit-identical frames.  Forced
     fallback through the grouped geometry equals the plain render.  Freezing
-    releases 16 bytes per table parameter and changes no frame."""
+    releases 16 bytes per table parameter and the training buffers, and changes
+    no frame."""
     vol32, host = small_vol
     vol64 = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=64)
     vol64.upload(host)
@@ -453,7 +466,9 @@ def test_grouped_bricks_and_freeze(ctx, small_vol):
     b0 = ctx.device_bytes()
     sg_g.freeze()
     freed = b0 - ctx.device_bytes()
-    assert freed == sg_g.nbricks * 16 * sg_g.T * 2 * 16 + 3 * ivr.SG_NPARAM * 4
+    # tables, gradients and moments (16 bytes per parameter), the MLP's gradient and
+    # moments, and the encoding-gradient buffer of a training chunk (2^16 samples x 32 fp32)
+    assert freed == sg_g.nbricks * 16 * sg_g.T * 2 * 16 + 3 * ivr.SG_NPARAM * 4 + (1 << 16) * 32 * 4
     sg_g.render(vol32, tf, d, fr)
     assert np.array_equal(fr.download(), frames[False])
     sg_g.render(None, tf, d, fr)  # no plain volume resident
