@@ -46,8 +46,12 @@ struct ivr_surrogate {
     size_t ntable = 0;
     float *tables = nullptr, *gtables = nullptr, *mt = nullptr, *vt = nullptr;
     void *tables16 = nullptr; /* half-precision copy of tables, read by rendering */
-    float *dE = nullptr;      /* training: encoding gradients of one chunk of samples */
-    size_t dE_n = 0;          /* samples dE holds */
+    float *dE = nullptr;      /* training: encoding gradients of two chunks of samples */
+    size_t dE_n = 0;          /* samples one chunk of dE holds */
+    /* training: the table scatter of a chunk runs on side while the next
+     * chunk's network pass runs on the context's stream */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_net[2] = {nullptr, nullptr}, ev_sc[2] = {nullptr, nullptr};
     float *params = nullptr, *gparams = nullptr, *mp = nullptr, *vp = nullptr;
     void *frag = nullptr;
     float *d_loss = nullptr;

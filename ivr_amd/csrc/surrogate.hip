@@ -947,6 +947,33 @@ int refresh_frags(ivr_ctx *ctx, ivr_surrogate *s)
 
 } // namespace
 
+namespace {
+
+/* the side stream and events of the training pipeline, made on first use */
+int train_streams(ivr_surrogate *s)
+{
+    if (s->side) return IVR_OK;
+    IVR_HIP(hipStreamCreateWithFlags(&s->side, hipStreamNonBlocking));
+    for (int q = 0; q < 2; q++) {
+        IVR_HIP(hipEventCreateWithFlags(&s->ev_net[q], hipEventDisableTiming));
+        IVR_HIP(hipEventCreateWithFlags(&s->ev_sc[q], hipEventDisableTiming));
+    }
+    return IVR_OK;
+}
+
+void train_streams_release(ivr_surrogate *s)
+{
+    for (int q = 0; q < 2; q++) {
+        if (s->ev_net[q]) (void)hipEventDestroy(s->ev_net[q]);
+        if (s->ev_sc[q]) (void)hipEventDestroy(s->ev_sc[q]);
+        s->ev_net[q] = s->ev_sc[q] = nullptr;
+    }
+    if (s->side) (void)hipStreamDestroy(s->side);
+    s->side = nullptr;
+}
+
+} // namespace
+
 bool ivr_surrogate_matches(const ivr_surrogate *s, const ivr_volume *v)
 {
     return v->n[0] == s->n[0] && v->n[1] == s->n[1] && v->n[2] == s->n[2] &&
@@ -1041,7 +1068,8 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
         ivr_dfree(ctx, s->d_fallback, IVR_MAX_BRICKS);
         ivr_dfree(ctx, s->d_cellrange, sizeof(float2) * s->ncells);
         ivr_dfree(ctx, s->d_cellempty, s->ncells);
-        ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
+        ivr_dfree(ctx, s->dE, 2 * s->dE_n * SG_IN * 4);
+        train_streams_release(s);
     }
     delete s;
     return IVR_OK;
@@ -1049,21 +1077,23 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
 
 namespace {
 
-/* samples per k_train_net / k_train_scatter launch pair: the chunk buffer of
- * encoding gradients is SG_CHUNK x 128 bytes (256 MiB), allocated at the
- * first training call */
+/* samples per k_train_net / k_train_scatter launch pair: the two chunk
+ * buffers of encoding gradients are 2 x SG_CHUNK x 128 bytes (512 MiB),
+ * allocated at the first training call */
 #define SG_CHUNK (1ll << 21)
 
 int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long long n, const float *xs, const int *bs,
                  const float *ts, float *ys)
 {
     const long long chunk = std::min<long long>(SG_CHUNK, (n + 63) / 64 * 64);
+    int rc;
+    if ((rc = train_streams(s))) return rc;
     if ((size_t)chunk > s->dE_n) {
-        ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
+        IVR_HIP(hipStreamSynchronize(s->side)); /* a previous step's scatter may still read dE */
+        ivr_dfree(ctx, s->dE, 2 * s->dE_n * SG_IN * 4);
         s->dE = nullptr;
         s->dE_n = 0;
-        int rc = ivr_dalloc(ctx, (void **)&s->dE, (size_t)chunk * SG_IN * 4);
-        if (rc) return rc;
+        if ((rc = ivr_dalloc(ctx, (void **)&s->dE, 2 * (size_t)chunk * SG_IN * 4))) return rc;
         s->dE_n = (size_t)chunk;
     }
     TrainArgs A;
@@ -1090,16 +1120,29 @@ int launch_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, long lon
     IVR_HIP(hipMemsetAsync(s->gtables, 0, s->ntable * 4, ctx->stream));
     IVR_HIP(hipMemsetAsync(s->gparams, 0, SG_NPARAM * 4, ctx->stream));
     IVR_HIP(hipMemsetAsync(s->d_loss, 0, 4, ctx->stream));
-    for (long long s0 = 0; s0 < n; s0 += chunk) {
+    /* chunk i: its network pass on the context's stream into dE buffer i % 2
+     * (after the scatter of chunk i - 2 has read that buffer), its scatter on
+     * the side stream once the network pass is done; the context's stream
+     * then waits for the last scatters */
+    int i = 0;
+    for (long long s0 = 0; s0 < n; s0 += chunk, i++) {
+        const int q = i & 1;
         A.s0 = s0;
         A.nchunk = std::min(chunk, n - s0);
+        A.dE = s->dE + (size_t)q * s->dE_n * SG_IN;
+        if (i >= 2) IVR_HIP(hipStreamWaitEvent(ctx->stream, s->ev_sc[q], 0));
         const long long groups = ((A.nchunk + 63) / 64 + 1) / 2;
         hipLaunchKernelGGL(k_train_net, dim3((unsigned)std::min<long long>(groups, 2048)), dim3(128), 0, ctx->stream, A);
+        IVR_HIP(hipGetLastError());
+        IVR_HIP(hipEventRecord(s->ev_net[q], ctx->stream));
+        IVR_HIP(hipStreamWaitEvent(s->side, s->ev_net[q], 0));
         const long long tiles = (A.nchunk + 63) / 64;
         hipLaunchKernelGGL(k_train_scatter, dim3((unsigned)std::min<long long>((tiles + SG_SC_WAVES - 1) / SG_SC_WAVES, 2048)),
-                           dim3(64 * SG_SC_WAVES), 0, ctx->stream, A);
+                           dim3(64 * SG_SC_WAVES), 0, s->side, A);
         IVR_HIP(hipGetLastError());
+        IVR_HIP(hipEventRecord(s->ev_sc[q], s->side));
     }
+    for (int q = 0; q < std::min(i, 2); q++) IVR_HIP(hipStreamWaitEvent(ctx->stream, s->ev_sc[q], 0));
     return IVR_OK;
 }
 
@@ -1625,9 +1668,10 @@ int ivr_surrogate_freeze(ivr_ctx *ctx, ivr_surrogate *s)
         ivr_dfree(ctx, *p, SG_NPARAM * 4);
         *p = nullptr;
     }
-    ivr_dfree(ctx, s->dE, s->dE_n * SG_IN * 4);
+    ivr_dfree(ctx, s->dE, 2 * s->dE_n * SG_IN * 4);
     s->dE = nullptr;
     s->dE_n = 0;
+    train_streams_release(s);
     s->frozen = true;
     return IVR_OK;
 }

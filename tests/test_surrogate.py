@@ -467,8 +467,8 @@ def test_grouped_bricks_and_freeze(ctx, small_vol):
     sg_g.freeze()
     freed = b0 - ctx.device_bytes()
     # tables, gradients and moments (16 bytes per parameter), the MLP's gradient and
-    # moments, and the encoding-gradient buffer of a training chunk (2^16 samples x 32 fp32)
-    assert freed == sg_g.nbricks * 16 * sg_g.T * 2 * 16 + 3 * ivr.SG_NPARAM * 4 + (1 << 16) * 32 * 4
+    # moments, and the encoding-gradient buffers of two training chunks (2^16 samples x 32 fp32)
+    assert freed == sg_g.nbricks * 16 * sg_g.T * 2 * 16 + 3 * ivr.SG_NPARAM * 4 + 2 * (1 << 16) * 32 * 4
     sg_g.render(vol32, tf, d, fr)
     assert np.array_equal(fr.download(), frames[False])
     sg_g.render(None, tf, d, fr)  # no plain volume resident
