@@ -230,8 +230,13 @@ void ivr_surrogate_defaults(ivr_surrogate_params *sp);
 int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogate_params *sp,
                          ivr_surrogate **out);
 int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s);
-/* `steps` Adam steps of `samples` uniform samples (spread over the bricks)
- * whose targets are the trilinear values of `vol`; loss_out = last MSE */
+/* `steps` Adam steps of `samples` samples whose targets are the trilinear
+ * values of `vol`; loss_out = last MSE.  The samples of a step go to the
+ * bricks in equal contiguous ranges, each uniform in its brick (stratified
+ * in tiles of 64), and are processed in chunks of up to 2^21 on the
+ * context's stream and one more stream of the surrogate's own; the first
+ * call allocates two chunk buffers of encoding gradients (128 B per sample
+ * of a chunk each), which ivr_surrogate_freeze releases. */
 int ivr_surrogate_train(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol, int steps,
                         long long samples, float *loss_out);
 /* gradients of one batch of caller samples (local xyz in [0,1]^3, brick id,
@@ -261,12 +266,14 @@ int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned ch
  * caller points: local xyz in [0,1]^3 of brick bricks[i]; y_out host, n floats */
 int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const float *xyz, const int *bricks,
                               float *y_out);
-/* versioned binary checkpoint of parameters and Adam state */
+/* versioned binary checkpoint of parameters and Adam state (format 4; files
+ * of formats 1-3 load too) */
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
 /* keep only what rendering reads: the half-precision tables, the MLP and the
  * macrocell ranges (2 bytes per table parameter); the fp32 tables, their
- * gradients and Adam moments (16 bytes per parameter) are released.  Render,
+ * gradients and Adam moments (16 bytes per parameter) and the training
+ * buffers and stream are released.  Render,
  * evaluate, eval_points, set_fallback and info work as before; train, grad,
  * adam, get, set, save and load then fail with IVR_ERR_UNSUPPORTED. */
 int ivr_surrogate_freeze(ivr_ctx *ctx, ivr_surrogate *s);

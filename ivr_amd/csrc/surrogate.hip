@@ -23,14 +23,18 @@
  *
  * Encoding work is spread the same way: lane l computes levels 4*(l/16) ..
  * 4*(l/16)+3 of the samples l%16, 16+l%16, 32+l%16, 48+l%16, which is the
- * B-operand fragment of the first layer.
+ * B-operand fragment of the first layer; the gathers of all four sample
+ * columns are in flight before any MLP work.
  *
- * The hash-table gradient (two fp32 atomics per corner, level and sample)
- * is scattered by 16 lanes per (sample, level): each lane adds one feature
- * of one of the cell's 8 corners, so the 64 atomics of an instruction fall
- * on the few cache lines of four cells (a 4x4x4 block keeps a cell's
- * corners within about 2.3 lines) rather than on 64 unrelated lines; the
- * training step went from 219 to 77 ms for 2^24 samples on 8 bricks.
+ * Training runs in chunks of samples, two kernels per chunk: k_train_net
+ * (encoding, forward, backward, weight gradients, and the gradient of the
+ * loss with respect to every sample's encoding, written to a chunk buffer)
+ * and k_train_scatter (that gradient added to the 8 corners of each cell at
+ * every level, two fp32 atomics per corner, level and sample, by 16 lanes
+ * per (sample, level) so that the 64 atomics of an instruction fall on the
+ * few cache lines of four cells; coarse levels whose vertices under a
+ * wavefront's samples form a small box are summed in LDS first).  A chunk's
+ * scatter runs on a second stream while the next chunk's network pass runs.
  */
 #include <cstddef>
 #include <cstring>
