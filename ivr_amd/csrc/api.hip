@@ -316,7 +316,8 @@ void ivr_fill_args(RenderArgs &a, const ivr_cam &cam, const ivr_volume *v, const
     a.P = v->P;
     a.nbricks = v->nbricks;
     a.vox_data = v->data;
-    a.vox_bytes = (unsigned long long)v->bytes < (1ull << 32) - (1ull << 20) ? (unsigned)v->bytes : 0u;
+    a.pair_data = v->pairs;
+    a.pair_bytes = 2ull * v->bytes < (1ull << 32) - (1ull << 20) ? (unsigned)(2 * v->bytes) : 0u;
     a.dtype_f16 = v->dtype == IVR_F16;
     a.boff = v->d_boff;
     a.cellempty = v->d_cellempty;

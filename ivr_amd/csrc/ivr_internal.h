@@ -93,6 +93,7 @@ struct ivr_volume {
     size_t stride = 0;                  /* elements per brick, brick_stride(P) */
     size_t bytes = 0;
     void *data = nullptr;               /* all bricks, Morton order            */
+    void *pairs = nullptr;              /* their row-pair copy (pair_elem), 2 x bytes */
     std::vector<unsigned long long> h_boff; /* linear brick id -> element offset */
     unsigned long long *d_boff = nullptr;
     int nc[3] = {0, 0, 0};              /* macrocells per axis                  */
@@ -148,7 +149,8 @@ struct RenderArgs {
     int bshift, bmask, P;
     int nbricks;
     const void *vox_data;
-    unsigned vox_bytes; /* bytes of vox_data when below 4 GiB - 1 MiB (buffer loads), else 0 */
+    const void *pair_data; /* the row-pair copy of the bricks (pair_elem) */
+    unsigned pair_bytes;   /* its bytes when below 4 GiB - 1 MiB (buffer loads), else 0 */
     int dtype_f16;
     const unsigned long long *boff;
     const unsigned char *cellempty;
@@ -185,6 +187,17 @@ __host__ __device__ static inline size_t brick_stride(int P)
 __host__ __device__ static inline unsigned brick_elem(int P, int x, int y, int z)
 {
     return ((unsigned)z * (unsigned)P + (unsigned)y) * (unsigned)P + (unsigned)x;
+}
+
+/* Row-pair copy of the bricks, read by the config-mode marches: brick b
+ * starts at element 2 * boff[b] (twice the brick storage, so no table of
+ * its own), and its element 2 (x + P (y + (P-1) z)) + dy holds the voxel at
+ * storage coordinates (x, y + dy, z), y < P-1.  The 8 voxels of a footprint
+ * (x..x+1, y..y+1, z..z+1) are then 4 consecutive elements in each of two
+ * slices: two loads instead of the four row loads of the brick storage. */
+__host__ __device__ static inline unsigned pair_elem(int P, int x, int y, int z)
+{
+    return 2u * (((unsigned)z * (unsigned)(P - 1) + (unsigned)y) * (unsigned)P + (unsigned)x);
 }
 
 /* inverse of brick_elem (padding elements decode to z >= P) */

@@ -225,60 +225,85 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs &a, const float *o, c
     return true;
 }
 
-/* The 8-voxel footprint of fetch8 through a raw buffer resource over the
- * volume: byte offset o of (j,i,k), cy and cz the byte strides of a row and
- * a slice.  Offsets at or past the resource's size return zeros without a
- * memory access, which lets a lane skip a footprint without a branch. */
+/* The 8-voxel footprint (fetch8's order) from the row-pair copy
+ * (pair_elem): 4 consecutive elements (x,y) (x,y+1) (x+1,y) (x+1,y+1) at
+ * byte offset o and the same at o + czp (the next slice), through a raw
+ * buffer resource over the copy (offsets at or past its size return zeros
+ * without a memory access), or from a pointer p. */
 template <typename VT>
-__device__ __forceinline__ void fetch8_buf(__amdgpu_buffer_rsrc_t rs, unsigned o, unsigned cy, unsigned cz,
-                                           float v[8])
+__device__ __forceinline__ void pair_unpack(const unsigned q0[4], const unsigned q1[4], float v[8])
 {
     if (sizeof(VT) == 4) {
-        typedef float f2 __attribute__((ext_vector_type(2)));
-        const f2 a0 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0));
-        const f2 a1 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cy, 0, 0));
-        const f2 a2 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cz, 0, 0));
-        const f2 a3 = __builtin_bit_cast(f2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + cy + cz, 0, 0));
-        v[0] = a0.x; v[1] = a0.y; v[2] = a1.x; v[3] = a1.y;
-        v[4] = a2.x; v[5] = a2.y; v[6] = a3.x; v[7] = a3.y;
+        v[0] = __uint_as_float(q0[0]); v[1] = __uint_as_float(q0[2]);
+        v[2] = __uint_as_float(q0[1]); v[3] = __uint_as_float(q0[3]);
+        v[4] = __uint_as_float(q1[0]); v[5] = __uint_as_float(q1[2]);
+        v[6] = __uint_as_float(q1[1]); v[7] = __uint_as_float(q1[3]);
     } else {
         typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-        const h2 a0 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o, 0, 0));
-        const h2 a1 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cy, 0, 0));
-        const h2 a2 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cz, 0, 0));
-        const h2 a3 = __builtin_bit_cast(h2, __builtin_amdgcn_raw_buffer_load_b32(rs, o + cy + cz, 0, 0));
-        v[0] = (float)a0.x; v[1] = (float)a0.y; v[2] = (float)a1.x; v[3] = (float)a1.y;
-        v[4] = (float)a2.x; v[5] = (float)a2.y; v[6] = (float)a3.x; v[7] = (float)a3.y;
+        const h2 a = __builtin_bit_cast(h2, q0[0]), b = __builtin_bit_cast(h2, q0[1]);
+        const h2 c = __builtin_bit_cast(h2, q1[0]), d = __builtin_bit_cast(h2, q1[1]);
+        v[0] = (float)a.x; v[1] = (float)b.x; v[2] = (float)a.y; v[3] = (float)b.y;
+        v[4] = (float)c.x; v[5] = (float)d.x; v[6] = (float)c.y; v[7] = (float)d.y;
     }
 }
 
-/* byte offset in the volume of the footprint origin of sample position q
- * (the buffer form of fetch8's address; the volume is below 4 GiB) */
 template <typename VT>
-__device__ __forceinline__ unsigned vox_offset(const RenderArgs &a, const unsigned long long *s_boff, const Pos &q)
+__device__ __forceinline__ void fetch8_pair_buf(__amdgpu_buffer_rsrc_t rs, unsigned o, unsigned czp, float v[8])
 {
-    const int bid = (q.j >> a.bshift) + a.nb[0] * ((q.i >> a.bshift) + a.nb[1] * (q.k >> a.bshift));
-    return (unsigned)(s_boff[bid] + brick_elem(a.P, (q.j & a.bmask) + 1, (q.i & a.bmask) + 1, (q.k & a.bmask) + 1)) *
-           (unsigned)sizeof(VT);
+    unsigned q0[4], q1[4];
+    if (sizeof(VT) == 4) {
+        typedef unsigned u4 __attribute__((ext_vector_type(4)));
+        const u4 x = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0));
+        const u4 y = __builtin_bit_cast(u4, __builtin_amdgcn_raw_buffer_load_b128(rs, o + czp, 0, 0));
+        q0[0] = x.x; q0[1] = x.y; q0[2] = x.z; q0[3] = x.w;
+        q1[0] = y.x; q1[1] = y.y; q1[2] = y.z; q1[3] = y.w;
+    } else {
+        typedef unsigned u2 __attribute__((ext_vector_type(2)));
+        const u2 x = __builtin_bit_cast(u2, __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0));
+        const u2 y = __builtin_bit_cast(u2, __builtin_amdgcn_raw_buffer_load_b64(rs, o + czp, 0, 0));
+        q0[0] = x.x; q0[1] = x.y; q1[0] = y.x; q1[1] = y.y;
+    }
+    pair_unpack<VT>(q0, q1, v);
 }
 
-/* Byte offset, modulo 2^32, of the footprint origin of sample positions
- * inside brick b, less esz * (j + P (i + P k)): with it the offset of (j,i,k)
- * is base + esz j + esz P i + esz P^2 k, two 24-bit multiply-adds (bricks
- * are at most 512 voxels, so esz P^2 < 2^24) instead of the brick lookup and
- * 32-bit products of vox_offset.  The true offset is below 2^32 (buffer
- * path), so the wrapped sum is exact. */
-__device__ __forceinline__ unsigned run_base(const RenderArgs &a, const unsigned long long *s_boff, const Pos &p0,
-                                             unsigned esz)
+template <typename VT>
+__device__ __forceinline__ void fetch8_pair_ptr(const char *p, unsigned czp, float v[8])
 {
-    /* brick of sample position p0 (shifts, no divisions: this runs once per ray and run) */
+    unsigned q0[4], q1[4];
+    __builtin_memcpy(q0, p, 4 * sizeof(VT));
+    __builtin_memcpy(q1, p + czp, 4 * sizeof(VT));
+    pair_unpack<VT>(q0, q1, v);
+}
+
+/* byte offset in the row-pair copy of the footprint of sample position q
+ * (from the brick offsets: brick b's copy starts at element 2 boff[b]);
+ * 64-bit for copies past 4 GiB */
+template <typename VT>
+__device__ __forceinline__ unsigned long long pair_offset(const RenderArgs &a, const unsigned long long *s_boff,
+                                                          const Pos &q)
+{
+    const int bid = (q.j >> a.bshift) + a.nb[0] * ((q.i >> a.bshift) + a.nb[1] * (q.k >> a.bshift));
+    return (2ull * s_boff[bid] + pair_elem(a.P, (q.j & a.bmask) + 1, (q.i & a.bmask) + 1, (q.k & a.bmask) + 1)) *
+           sizeof(VT);
+}
+
+/* For the footprints of one brick (a sort-last run, first sample p0): the
+ * brick's byte offset in the row-pair copy, and loc0 such that the offset
+ * of (j,i,k) inside it is loc0 + 2 esz j + cyp i + czp k modulo 2^32 (two
+ * 24-bit multiply-adds: cyp = 2 esz P and czp = 2 esz P (P-1) are below
+ * 2^24 for bricks of at most 512 voxels).  The true offset inside a brick is
+ * below 2^32, so the wrapped sum is exact. */
+__device__ __forceinline__ void pair_run(const RenderArgs &a, const unsigned long long *s_boff, const Pos &p0,
+                                         unsigned esz, unsigned long long &brick, unsigned &loc0)
+{
     const int bx = p0.j >> a.bshift, by = p0.i >> a.bshift, bz = p0.k >> a.bshift;
     const int b = bx + (int)__umul24((unsigned)a.nb[0], (unsigned)(by + (int)__umul24((unsigned)a.nb[1], (unsigned)bz)));
-    const int ox = 1 - (bx << a.bshift), oy = 1 - (by << a.bshift), oz = 1 - (bz << a.bshift);
-    /* ox + P (oy + P oz), modulo 2^32 like the rest */
-    const unsigned lo = (unsigned)ox + (unsigned)a.P * ((unsigned)oy + (unsigned)a.P * (unsigned)oz);
-    return ((unsigned)s_boff[b] + lo) * esz;
+    brick = 2ull * s_boff[b] * esz;
+    const unsigned ox = 1u - ((unsigned)bx << a.bshift), oy = 1u - ((unsigned)by << a.bshift),
+                   oz = 1u - ((unsigned)bz << a.bshift);
+    loc0 = 2u * esz * (ox + (unsigned)a.P * (oy + (unsigned)(a.P - 1) * oz));
 }
+
 
 /* march_segment without macrocell skipping, software-pipelined: the next
  * sample's footprint is requested before the current sample is composited,
@@ -293,23 +318,22 @@ __device__ __forceinline__ void march_segment_pipe(const RenderArgs &a, const fl
                                                    int k1, Acc &acc, unsigned &cnt, unsigned &fetched)
 {
     if (k0 >= k1) return;
+    /* footprints from the row-pair copy: two loads each instead of four
+     * (C2 sort-last k_runs 1.05 -> 0.79 ms in a probe issuing two) */
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)a.vox_data, (short)0, (int)a.vox_bytes, 0x00020000);
-    const unsigned esz = sizeof(VT), cy = (unsigned)a.P * esz, cz = (unsigned)(a.P * a.P) * esz;
-    /* the brick-local addressing of run_base was measured slower here (C2
-     * sort-last 1.087 vs 1.054 ms: 13.5 % fewer VALU instructions, 8 % more
-     * cycles waiting on the footprint loads), so this march keeps vox_offset */
-    auto offset = [&](const Pos &q) { return vox_offset<VT>(a, s_boff, q); };
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.pair_data, (short)0, (int)a.pair_bytes, 0x00020000);
+    const unsigned esz = sizeof(VT), czp = 2u * esz * (unsigned)a.P * (unsigned)(a.P - 1);
+    auto offset = [&](const Pos &q) { return (unsigned)pair_offset<VT>(a, s_boff, q); };
     int k = k0;
     Pos p = cfg_pos(a, r.go, r.gd, r.tin, r.dt, k);
     float v[8];
-    fetch8_buf<VT>(rs, offset(p), cy, cz, v);
+    fetch8_pair_buf<VT>(rs, offset(p), czp, v);
     for (;;) {
         const bool more = k + 1 < k1;
         const Pos pn = cfg_pos(a, r.go, r.gd, r.tin, r.dt, more ? k + 1 : k);
         const bool need = more && (pn.j != p.j || pn.i != p.i || pn.k != p.k);
         float vn[8];
-        fetch8_buf<VT>(rs, need ? offset(pn) : a.vox_bytes, cy, cz, vn);
+        fetch8_pair_buf<VT>(rs, need ? offset(pn) : a.pair_bytes, czp, vn);
         __builtin_amdgcn_sched_barrier(0); /* keep the requests ahead of the compositing */
         ftb_sample(a, s_lut, v, p, acc);
         cnt++;
@@ -327,24 +351,27 @@ __device__ __forceinline__ void march_segment_pipe(const RenderArgs &a, const fl
  * wavefront the rays crossing empty cells and the rays sampling advance
  * together.  cnt counts composited and skipped samples, fetched the
  * samples that were classified.  RUNB: every sample of the segment lies in
- * one brick (a sort-last run), whose footprints are addressed from run_base
- * (C3 sort-last k_runs 1.12 -> 1.10 ms). */
+ * one brick (a sort-last run), whose footprints are addressed inside it
+ * (pair_run). */
 template <typename VT, bool SKIP, bool RUNB = false>
 __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 *s_lut,
                                               const unsigned long long *s_boff, const Ray &r, int k0,
                                               int k1, Acc &acc, unsigned &cnt, unsigned &fetched)
 {
-    if (!SKIP && a.vox_bytes) {
+    if (!SKIP && a.pair_bytes) {
         march_segment_pipe<VT>(a, s_lut, s_boff, r, k0, k1, acc, cnt, fetched);
         return;
     }
-    /* footprints through the buffer resource (two voxels per load) when the
-     * volume allows it (uniform per launch), else through plain pointers */
-    const bool buf = a.vox_bytes != 0;
+    /* footprints from the row-pair copy, two loads each, through the buffer
+     * resource when the copy is below 4 GiB (uniform per launch), else through
+     * pointers; a run's footprints addressed inside its brick (pair_run) */
+    const bool buf = a.pair_bytes != 0;
     const __amdgpu_buffer_rsrc_t rs =
-        __builtin_amdgcn_make_buffer_rsrc((void *)a.vox_data, (short)0, (int)a.vox_bytes, 0x00020000);
-    const unsigned esz = sizeof(VT), cy = (unsigned)a.P * esz, cz = (unsigned)(a.P * a.P) * esz;
-    const unsigned rbase = RUNB && buf && k0 < k1 ? run_base(a, s_boff, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k0), esz) : 0u;
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.pair_data, (short)0, (int)a.pair_bytes, 0x00020000);
+    const unsigned esz = sizeof(VT), cyp = 2u * esz * (unsigned)a.P, czp = cyp * (unsigned)(a.P - 1);
+    unsigned long long rbrick = 0;
+    unsigned rloc0 = 0;
+    if (RUNB && k0 < k1) pair_run(a, s_boff, cfg_pos(a, r.go, r.gd, r.tin, r.dt, k0), esz, rbrick, rloc0);
     int k = k0;
     int cj = -1, ci = -1, ck = -1; /* floor index of the voxels held in v */
     int cc = -1, e = 0;            /* macrocell of the code held in e      */
@@ -365,12 +392,14 @@ __device__ __forceinline__ void march_segment(const RenderArgs &a, const float4 
             }
         }
         if (p.j != cj || p.i != ci || p.k != ck) { /* consecutive samples often share their 8 voxels */
-            if (buf) {
-                const unsigned o = RUNB ? __umul24((unsigned)p.k, cz) + (__umul24((unsigned)p.i, cy) + ((unsigned)p.j * esz + rbase))
-                                        : vox_offset<VT>(a, s_boff, p);
-                fetch8_buf<VT>(rs, o, cy, cz, v);
+            if (RUNB) {
+                const unsigned o = __umul24((unsigned)p.k, czp) + (__umul24((unsigned)p.i, cyp) + ((unsigned)p.j * (2u * esz) + rloc0));
+                if (buf) fetch8_pair_buf<VT>(rs, (unsigned)rbrick + o, czp, v);
+                else fetch8_pair_ptr<VT>((const char *)a.pair_data + rbrick + o, czp, v);
             } else {
-                fetch8<VT>(a, s_boff, p.j, p.i, p.k, v);
+                const unsigned long long o = pair_offset<VT>(a, s_boff, p);
+                if (buf) fetch8_pair_buf<VT>(rs, (unsigned)o, czp, v);
+                else fetch8_pair_ptr<VT>((const char *)a.pair_data + o, czp, v);
             }
             cj = p.j;
             ci = p.i;

@@ -79,6 +79,23 @@ __global__ void k_scatter(BrickGeom g, const unsigned long long *boff, const flo
     dst[boff[bid] + e] = (VT)v;
 }
 
+/* the row-pair copy of brick blockIdx.y (pair_elem): one thread per
+ * (x, y, z), y < P-1, reading voxels (x, y, z) and (x, y+1, z) of the brick
+ * storage and writing them side by side */
+template <typename VT>
+__global__ void k_pairs(BrickGeom g, const unsigned long long *boff, const VT *src, VT *dst)
+{
+    const int P = g.P;
+    const unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (unsigned)P * (unsigned)(P - 1) * (unsigned)P) return;
+    const int x = (int)(e % (unsigned)P), y = (int)((e / (unsigned)P) % (unsigned)(P - 1)),
+              z = (int)(e / ((unsigned)P * (unsigned)(P - 1)));
+    const VT *b = src + boff[blockIdx.y];
+    VT *d = dst + 2 * boff[blockIdx.y] + pair_elem(P, x, y, z);
+    d[0] = b[brick_elem(P, x, y, z)];
+    d[1] = b[brick_elem(P, x, y + 1, z)];
+}
+
 /* bricks -> linear fp32 */
 template <typename VT>
 __global__ void k_gather(BrickGeom g, const unsigned long long *boff, const VT *src, float *lin)
@@ -379,7 +396,7 @@ int ivr_volume_create(ivr_ctx *ctx, int nx, int ny, int nz, int dtype, int brick
     for (int r = 0; r < 3; r++) v->nc[r] = ((v->n[r] - 2) >> IVR_CELL_SHIFT) + 1;
     v->ncells = (size_t)v->nc[0] * v->nc[1] * v->nc[2];
     int rc;
-    if ((rc = ivr_dalloc(ctx, &v->data, v->bytes)) ||
+    if ((rc = ivr_dalloc(ctx, &v->data, v->bytes)) || (rc = ivr_dalloc(ctx, &v->pairs, 2 * v->bytes)) ||
         (rc = ivr_dalloc(ctx, (void **)&v->d_boff, sizeof(unsigned long long) * v->nbricks)) ||
         (rc = ivr_dalloc(ctx, (void **)&v->d_cellrange, sizeof(float2) * v->ncells)) ||
         (rc = ivr_dalloc(ctx, (void **)&v->d_cellempty, v->ncells))) {
@@ -400,6 +417,7 @@ int ivr_volume_destroy(ivr_ctx *ctx, ivr_volume *v)
         (void)hipStreamSynchronize(ctx->stream);
         const size_t esz = v->dtype == IVR_F16 ? 2 : 4;
         ivr_dfree(ctx, v->data, v->stride * v->nbricks * esz);
+        ivr_dfree(ctx, v->pairs, 2 * v->stride * v->nbricks * esz);
         ivr_dfree(ctx, v->d_boff, sizeof(unsigned long long) * v->nbricks);
         ivr_dfree(ctx, v->d_cellrange, sizeof(float2) * v->ncells);
         ivr_dfree(ctx, v->d_cellempty, v->ncells);
@@ -436,6 +454,22 @@ int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *v)
     return IVR_OK;
 }
 
+/* the row-pair copy of every brick, after the bricks were written */
+int ivr_volume_pairs_build(ivr_ctx *ctx, ivr_volume *v)
+{
+    const BrickGeom g = geom_of(v);
+    const unsigned n = (unsigned)v->P * (unsigned)(v->P - 1) * (unsigned)v->P;
+    const dim3 grid((n + 255) / 256, (unsigned)v->nbricks);
+    if (v->dtype == IVR_F16)
+        hipLaunchKernelGGL(k_pairs<h16>, grid, dim3(256), 0, ctx->stream, g, v->d_boff, (const h16 *)v->data,
+                           (h16 *)v->pairs);
+    else
+        hipLaunchKernelGGL(k_pairs<float>, grid, dim3(256), 0, ctx->stream, g, v->d_boff, (const float *)v->data,
+                           (float *)v->pairs);
+    IVR_HIP(hipGetLastError());
+    return IVR_OK;
+}
+
 int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *v, const float *host)
 {
     if (!ctx || !v || !host) return ivr_fail(IVR_ERR_ARG, "null argument");
@@ -453,7 +487,7 @@ int ivr_volume_upload(ivr_ctx *ctx, ivr_volume *v, const float *host)
         hipLaunchKernelGGL(k_scatter<float>, grid, dim3(256), 0, ctx->stream, g, v->d_boff, lin,
                            (float *)v->data);
     IVR_HIP(hipGetLastError());
-    if ((rc = ivr_volume_cells_build(ctx, v))) return rc;
+    if ((rc = ivr_volume_cells_build(ctx, v)) || (rc = ivr_volume_pairs_build(ctx, v))) return rc;
     IVR_HIP(hipStreamSynchronize(ctx->stream));
     return IVR_OK;
 }
@@ -572,7 +606,7 @@ int ivr_volume_generate(ivr_ctx *ctx, ivr_volume *v, const ivr_gen_params *gp)
         hipLaunchKernelGGL(k_normalise<h16>, dim3(nbn), dim3(256), 0, ctx->stream, g, raw, (h16 *)v->data,
                            total_e, gp->kind, lo, hi, scale);
     IVR_HIP(hipGetLastError());
-    if ((rc = ivr_volume_cells_build(ctx, v))) return rc;
+    if ((rc = ivr_volume_cells_build(ctx, v)) || (rc = ivr_volume_pairs_build(ctx, v))) return rc;
     IVR_HIP(hipStreamSynchronize(ctx->stream));
     return IVR_OK;
 }
