@@ -266,12 +266,14 @@ int ivr_surrogate_set_fallback(ivr_ctx *ctx, ivr_surrogate *s, const unsigned ch
  * caller points: local xyz in [0,1]^3 of brick bricks[i]; y_out host, n floats */
 int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const float *xyz, const int *bricks,
                               float *y_out);
-/* versioned binary checkpoint of parameters and Adam state (format 4; files
- * of formats 1-3 load too) */
+/* versioned binary checkpoint of parameters, Adam state and the 4^3-voxel
+ * macrocell ranges (format 5; files of formats 1-4 load too, their ranges of
+ * 16^3-voxel cells are dropped for the ones computed at create) */
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
 /* keep only what rendering reads: the half-precision tables, the MLP and the
- * macrocell ranges (2 bytes per table parameter); the fp32 tables, their
+ * 4^3-voxel macrocell ranges (2 bytes per table parameter, 9 per 64
+ * voxels); the fp32 tables, their
  * gradients and Adam moments (16 bytes per parameter) and the training
  * buffers and stream are released.  Render,
  * evaluate, eval_points, set_fallback and info work as before; train, grad,

@@ -239,5 +239,8 @@ int ivr_colormap_apply(hipStream_t st, float4 *d_lut, int id, int n, float hmin,
 
 /* volume kernels (volume.hip) */
 int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *vol);
+/* value range (with the trilinear footprint) of every cell of 2^shift voxels
+ * per axis, nc cells per axis ((n - 2) >> shift) + 1, into out (device) */
+int ivr_volume_cellrange(ivr_ctx *ctx, const ivr_volume *vol, int shift, const int nc[3], float2 *out);
 
 #endif

@@ -128,13 +128,15 @@ __device__ __forceinline__ void ray_voxel(const RenderArgs &a, const float *o, c
     }
 }
 
+/* CS: log2 of the macrocell edge in voxels (IVR_CELL_SHIFT for the plain
+ * marches; the surrogate renderer uses finer cells, SG_CELL_SHIFT) */
+template <int CS = IVR_CELL_SHIFT>
 __device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
 {
     /* on the full-rate 24-bit multiplier: the cell counts per axis and the
      * partial sums are below 2^24, the index below 2^32 */
-    return (int)((unsigned)(p.j >> IVR_CELL_SHIFT) +
-                 __umul24((unsigned)a.nc[0], (unsigned)(p.i >> IVR_CELL_SHIFT) +
-                                                 __umul24((unsigned)a.nc[1], (unsigned)(p.k >> IVR_CELL_SHIFT))));
+    return (int)((unsigned)(p.j >> CS) +
+                 __umul24((unsigned)a.nc[0], (unsigned)(p.i >> CS) + __umul24((unsigned)a.nc[1], (unsigned)(p.k >> CS))));
 }
 
 /* First sample index past k on the grid t_k = tin + k dt whose cell may be
@@ -146,11 +148,12 @@ __device__ __forceinline__ int cell_of(const RenderArgs &a, const Pos &p)
  * between lie in the node.  The result is at most kmax.  Only samples of
  * transparent cells are passed, whatever the estimate: it decides how many
  * steps a skip takes, never which samples are composited. */
+template <int CS = IVR_CELL_SHIFT>
 __device__ __forceinline__ int skip_empty(const RenderArgs &a, const float *go, const float *gd,
                                           const float *rgd, float tin, float dt, float rdt, const Pos &p,
                                           int e, int k, int kmax)
 {
-    const int sh = IVR_CELL_SHIFT - 1 + e;
+    const int sh = CS - 1 + e;
     const int cc[3] = {p.j >> sh, p.i >> sh, p.k >> sh};
     float te = FLT_MAX;
 #pragma unroll

@@ -28,6 +28,12 @@
 #define SG_FRAG_G1 20 /* W1^T, 2 x 2, permuted                 */
 #define SG_NFRAG 24
 
+/* macrocells of the surrogate renderer: 4^3 voxels (the plain marches use
+ * 16^3), so that fewer samples that cannot contribute reach the network: a
+ * network sample costs tens of times a plain one, the finer cells 9 bytes
+ * per 64 voxels */
+#define SG_CELL_SHIFT 2
+
 /* bytes allocated past the half-precision tables: the encoding reads every
  * x-neighbour pair of corners with one 8-byte load, which at the last entry
  * of the last table reads 4 bytes beyond it (and discards them) */
@@ -57,9 +63,10 @@ struct ivr_surrogate {
     float *d_loss = nullptr;
     unsigned char *d_fallback = nullptr;
     int step = 0;
-    /* value range of every 16^3 macrocell of the trained volume (2 floats
-     * per 4096 voxels): the surrogate skips cells that the frame's transfer
-     * function makes transparent, as the plain renderer does */
+    /* value range of every 4^3 macrocell (SG_CELL_SHIFT) of the trained
+     * volume (2 floats per 64 voxels): the surrogate skips cells that the
+     * frame's transfer function makes transparent, as the plain renderer
+     * does with its 16^3 cells */
     int nc[3] = {0, 0, 0};
     size_t ncells = 0;
     float2 *d_cellrange = nullptr;

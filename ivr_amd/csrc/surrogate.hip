@@ -1006,8 +1006,11 @@ int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogat
         s->dense_n1[l] = n1 * n1 * n1 <= s->T ? (unsigned)n1 : 0u;
     }
     s->ntable = (size_t)s->nbricks * SG_L * s->T * SG_F;
-    for (int r = 0; r < 3; r++) s->nc[r] = vol->nc[r];
-    s->ncells = vol->ncells;
+    s->ncells = 1;
+    for (int r = 0; r < 3; r++) {
+        s->nc[r] = ((vol->n[r] - 2) >> SG_CELL_SHIFT) + 1;
+        s->ncells *= (size_t)s->nc[r];
+    }
     int rc;
     if ((rc = ivr_dalloc(ctx, (void **)&s->tables, s->ntable * 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->tables16, s->ntable * 2 + SG_T16_TAIL)) ||
@@ -1021,13 +1024,15 @@ int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogat
         (rc = ivr_dalloc(ctx, (void **)&s->frag, SG_NFRAG * 64 * 16)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->d_loss, 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->d_fallback, IVR_MAX_BRICKS)) ||
-        (rc = ivr_dalloc(ctx, (void **)&s->d_cellrange, sizeof(float2) * vol->ncells)) ||
-        (rc = ivr_dalloc(ctx, (void **)&s->d_cellempty, vol->ncells))) {
+        (rc = ivr_dalloc(ctx, (void **)&s->d_cellrange, sizeof(float2) * s->ncells)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->d_cellempty, s->ncells))) {
         ivr_surrogate_destroy(ctx, s);
         return rc;
     }
-    IVR_HIP(hipMemcpyAsync(s->d_cellrange, vol->d_cellrange, sizeof(float2) * vol->ncells,
-                           hipMemcpyDeviceToDevice, ctx->stream));
+    if ((rc = ivr_volume_cellrange(ctx, vol, SG_CELL_SHIFT, s->nc, s->d_cellrange))) {
+        ivr_surrogate_destroy(ctx, s);
+        return rc;
+    }
     const unsigned seed = (unsigned)(sp->seed ^ (sp->seed >> 32));
     hipLaunchKernelGGL(k_init, dim3((unsigned)((s->ntable + 255) / 256)), dim3(256), 0, ctx->stream, s->tables,
                        s->ntable, sp->init_scale, seed);
@@ -1345,13 +1350,13 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
         if (SKIP && live) {
             while (k < n) {
                 p = cfg_pos(a, go, gd, tin, dt, k);
-                const int cell = cell_of(a, p);
+                const int cell = cell_of<SG_CELL_SHIFT>(a, p);
                 if (cell != cc) {
                     cc = cell;
                     e = a.cellempty[cell];
                 }
                 if (!e) break;
-                const int kc = skip_empty(a, go, gd, rgd, tin, dt, rdt, p, e, k, n);
+                const int kc = skip_empty<SG_CELL_SHIFT>(a, go, gd, rgd, tin, dt, rdt, p, e, k, n);
                 cnt += (unsigned)(kc - k);
                 k = kc;
             }
@@ -1728,9 +1733,11 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
     return ivr_surrogate_set_fallback(ctx, s, flags.data());
 }
 
-/* checkpoint: "IVRSG004", params struct, dims, step, then MLP (params, m, v),
- * tables (values, m, v) and the macrocell ranges as little-endian float32.
- * Older files still load: "IVRSG003" and before stored a hashed level's
+/* checkpoint: "IVRSG005", params struct, dims, step, then MLP (params, m, v),
+ * tables (values, m, v) and the 4^3-voxel macrocell ranges as little-endian
+ * float32. Older files still load: "IVRSG004" and before hold ranges of
+ * 16^3-voxel cells, which are read and dropped (the ranges computed from the
+ * volume at create stay); "IVRSG003" and before stored a hashed level's
  * 4x4x4 block Morton-ordered in its slot (x0 y0 z0 x1 y1 z1), which is
  * permuted to the current order on load; "IVRSG002" files have a params
  * struct without log2_group (then 0), "IVRSG001" files no ranges either. */
@@ -1770,7 +1777,7 @@ int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
     if (s->frozen) return ivr_fail(IVR_ERR_UNSUPPORTED, "the surrogate is frozen (no training state)");
     FILE *f = fopen(path, "wb");
     if (!f) return ivr_fail(IVR_ERR_IO, "cannot open %s", path);
-    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '4'};
+    const char magic[8] = {'I', 'V', 'R', 'S', 'G', '0', '0', '5'};
     int hdr[8] = {s->nbricks, s->B, s->n[0], s->n[1], s->n[2], s->step, (int)s->T, SG_NPARAM};
     bool ok = fwrite(magic, 8, 1, f) == 1 && fwrite(&s->sp, sizeof s->sp, 1, f) == 1 &&
               fwrite(hdr, sizeof hdr, 1, f) == 1;
@@ -1795,7 +1802,7 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     char magic[8];
     ivr_surrogate_params sp;
     int hdr[8];
-    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG00", 7) && magic[7] >= '1' && magic[7] <= '4';
+    bool ok = fread(magic, 8, 1, f) == 1 && !memcmp(magic, "IVRSG00", 7) && magic[7] >= '1' && magic[7] <= '5';
     /* versions 1 and 2 stored the struct up to seed */
     const size_t sp_bytes = ok && magic[7] < '3' ? offsetof(ivr_surrogate_params, log2_group) : sizeof sp;
     memset(&sp, 0, sizeof sp);
@@ -1818,7 +1825,14 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     /* every section is read into host memory before the device state is touched,
      * so a short or failed read leaves the surrogate as it was */
     float *dev[7] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt, (float *)s->d_cellrange};
-    const size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, 2 * s->ncells};
+    size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, 2 * s->ncells};
+    /* files before "IVRSG005" hold ranges of 16^3-voxel cells */
+    const bool old_ranges = nsec == 7 && magic[7] < '5';
+    if (old_ranges) {
+        size_t n16 = 1;
+        for (int r = 0; r < 3; r++) n16 *= (size_t)(((s->n[r] - 2) >> 4) + 1);
+        cnt[6] = 2 * n16;
+    }
     std::vector<std::vector<float>> buf(nsec);
     for (int k = 0; k < nsec && ok; k++) {
         buf[k].resize(cnt[k]);
@@ -1828,7 +1842,7 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     if (!ok) return ivr_fail(IVR_ERR_IO, "reading %s failed", path);
     if (magic[7] < '4') /* Morton-ordered slots of the hashed levels to the current order */
         for (int k = 3; k < 6; k++) morton_slots_to_rows(s, buf[k].data());
-    for (int k = 0; k < nsec; k++)
+    for (int k = 0; k < (old_ranges ? 6 : nsec); k++)
         IVR_HIP(hipMemcpy(dev[k], buf[k].data(), cnt[k] * 4, hipMemcpyHostToDevice));
     s->sp = sp;
     s->step = hdr[5];

@@ -256,7 +256,7 @@ __global__ void k_normalise(BrickGeom g, const float *src, VT *dst, size_t total
 /* per-cell min/max over the footprint voxels [16c, min(16c+16, n-1)] */
 template <typename VT>
 __global__ void k_cellrange(BrickGeom g, const unsigned long long *boff, const VT *src, int nc0,
-                            int nc1, int nc2, float2 *out)
+                            int nc1, int nc2, int shift, float2 *out)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc0 * nc1 * nc2) return;
@@ -264,8 +264,8 @@ __global__ void k_cellrange(BrickGeom g, const unsigned long long *boff, const V
     const int c3[3] = {cx, cy, cz};
     int lo[3], hi[3];
     for (int r = 0; r < 3; r++) {
-        lo[r] = c3[r] << IVR_CELL_SHIFT;
-        hi[r] = min(lo[r] + (1 << IVR_CELL_SHIFT), g.n[r] - 1);
+        lo[r] = c3[r] << shift;
+        hi[r] = min(lo[r] + (1 << shift), g.n[r] - 1);
     }
     float mn = FLT_MAX, mx = -FLT_MAX;
     for (int z = lo[2]; z <= hi[2]; z++)
@@ -440,18 +440,24 @@ int ivr_volume_info(const ivr_volume *v, int dims[3], int nbricks[3], int *brick
     return IVR_OK;
 }
 
-int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *v)
+int ivr_volume_cellrange(ivr_ctx *ctx, const ivr_volume *v, int shift, const int nc[3], float2 *out)
 {
     const BrickGeom g = geom_of(v);
-    const unsigned nb = (unsigned)((v->ncells + 127) / 128);
+    const size_t n = (size_t)nc[0] * nc[1] * nc[2];
+    const unsigned nb = (unsigned)((n + 127) / 128);
     if (v->dtype == IVR_F16)
         hipLaunchKernelGGL(k_cellrange<h16>, dim3(nb), dim3(128), 0, ctx->stream, g, v->d_boff,
-                           (const h16 *)v->data, v->nc[0], v->nc[1], v->nc[2], v->d_cellrange);
+                           (const h16 *)v->data, nc[0], nc[1], nc[2], shift, out);
     else
         hipLaunchKernelGGL(k_cellrange<float>, dim3(nb), dim3(128), 0, ctx->stream, g, v->d_boff,
-                           (const float *)v->data, v->nc[0], v->nc[1], v->nc[2], v->d_cellrange);
+                           (const float *)v->data, nc[0], nc[1], nc[2], shift, out);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
+}
+
+int ivr_volume_cells_build(ivr_ctx *ctx, ivr_volume *v)
+{
+    return ivr_volume_cellrange(ctx, v, IVR_CELL_SHIFT, v->nc, v->d_cellrange);
 }
 
 /* the row-pair copy of every brick, after the bricks were written */

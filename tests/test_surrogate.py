@@ -330,11 +330,13 @@ def test_gpu_gradients_match_oracle(ctx, small_vol, layout):
 
 
 def _write_morton_checkpoint(src, dst, sg):
-    """rewrite checkpoint src (format 4) as format 3: the same sections with
-    every hashed level's slots in Morton order, the layout of that format"""
+    """rewrite checkpoint src (format 5) as format 3: the same sections with
+    every hashed level's slots in Morton order and macrocell ranges over
+    16^3-voxel cells, the layout of that format"""
     import ctypes
     raw = bytearray(open(src, "rb").read())
-    assert raw[:8] == b"IVRSG004"
+    assert raw[:8] == b"IVRSG005"
+    hdr = np.frombuffer(raw, np.int32, 8, 8 + ctypes.sizeof(ivr.SurrogateParams))
     off = 8 + ctypes.sizeof(ivr.SurrogateParams) + 8 * 4
     nparam = 6337
     ntab = sg.nbricks * 16 * sg.T * 2
@@ -352,6 +354,12 @@ def _write_morton_checkpoint(src, dst, sg):
             if hashed[lvl]:
                 t[:, lvl] = t[:, lvl][:, :, to, :]
         raw[o:o + 4 * ntab] = t.tobytes()
+    ranges = off + 4 * (3 * nparam + 3 * ntab)
+    n4 = int(np.prod([((int(n) - 2) >> 2) + 1 for n in hdr[2:5]]))
+    n16 = int(np.prod([((int(n) - 2) >> 4) + 1 for n in hdr[2:5]]))
+    assert len(raw) == ranges + 4 * 2 * n4
+    # ranges of 16^3 cells (the full range [0, 1] each): the loader reads and drops them
+    raw = raw[:ranges] + np.tile(np.float32([0.0, 1.0]), n16).tobytes()
     raw[:8] = b"IVRSG003"
     open(dst, "wb").write(bytes(raw))
 
