@@ -272,7 +272,7 @@ int ivr_surrogate_eval_points(ivr_ctx *ctx, const ivr_surrogate *s, int n, const
 int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path);
 int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path);
 /* keep only what rendering reads: the half-precision tables, the MLP and the
- * 4^3-voxel macrocell ranges (2 bytes per table parameter, 9 per 64
+ * 4^3-voxel macrocell ranges (2 bytes per table parameter, 4 per 64
  * voxels); the fp32 tables, their
  * gradients and Adam moments (16 bytes per parameter) and the training
  * buffers and stream are released.  Render,

@@ -299,7 +299,7 @@ int ivr_batcher_add_surrogate(ivr_batcher *b, const ivr_surrogate *sg, const ivr
      * fragments, cell ranges */
     const unsigned long long bytes = sg->ntable * (sg->frozen ? 2ull : 18ull) +
                                      SG_NPARAM * (sg->frozen ? 4ull : 16ull) + SG_NFRAG * 1024ull +
-                                     sg->ncells * 9ull + sg->nbricks;
+                                     sg->ncells * 5ull + sg->nbricks;
     if (b->used + bytes > b->budget)
         return ivr_fail(IVR_ERR_BUDGET, "surrogate of %llu bytes exceeds the HBM budget", bytes);
     Resident r;

@@ -224,6 +224,9 @@ int ivr_launch_render(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int sk
 int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a);
 /* transparency codes (see ivr_volume::d_cellempty) of the frame's LUT into out, ncells bytes */
 int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out);
+/* the same from half-precision ranges of nc[0] x nc[1] x nc[2] cells */
+int ivr_launch_cells_half(hipStream_t stream, const __half2 *range, const int nc[3], const RenderArgs &a,
+                          unsigned char *out);
 int ivr_launch_tile(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a);
 /* sort-last with one workgroup per (pixel block, run): partial planes into a.tile, then combined */
 int ivr_launch_runs(ivr_ctx *ctx, const ivr_volume *vol, RenderArgs &a, int skip);

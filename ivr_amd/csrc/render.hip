@@ -294,9 +294,15 @@ __device__ __forceinline__ bool cell_transparent(float2 r, const float4 *lut, in
     return true;
 }
 
-__global__ __launch_bounds__(512) void k_cells(const float2 *range, unsigned char *code, int ncx, int ncy,
-                                               int ncz, const float4 *lut, int lutsize, float lutscale,
-                                               int lutoffset, float alpha)
+__device__ __forceinline__ float2 range_of(float2 r) { return r; }
+__device__ __forceinline__ float2 range_of(__half2 r) { return __half22float2(r); }
+
+/* RT: float2 ranges of a volume's cells, or __half2 ranges (rounded outward
+ * when stored) of a surrogate's cells */
+template <typename RT>
+__global__ __launch_bounds__(512) void k_cells(const RT *range, unsigned char *code, int ncx, int ncy, int ncz,
+                                               const float4 *lut, int lutsize, float lutscale, int lutoffset,
+                                               float alpha)
 {
     __shared__ unsigned char s0[512], s1[64], s2[8], s3;
     const int t = threadIdx.x;
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(512) void k_cells(const float2 *range, unsigned cha
     const int x = blockIdx.x * 8 + tx, y = blockIdx.y * 8 + ty, z = blockIdx.z * 8 + tz;
     const bool in = x < ncx && y < ncy && z < ncz;
     const size_t c = (size_t)x + (size_t)ncx * ((size_t)y + (size_t)ncy * z);
-    const bool e = !in || cell_transparent(range[in ? c : 0], lut, lutsize, lutscale, lutoffset, alpha);
+    const bool e = !in || cell_transparent(range_of(range[in ? c : 0]), lut, lutsize, lutscale, lutoffset, alpha);
     s0[t] = e;
     __syncthreads();
     if (t < 64) {
@@ -481,14 +487,30 @@ int ivr_launch_cells(ivr_ctx *ctx, const ivr_volume *vol, const RenderArgs &a)
     return ivr_launch_cells_on(ctx->stream, vol, a, vol->d_cellempty);
 }
 
-int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out)
+namespace {
+
+template <typename RT>
+int launch_cells(hipStream_t stream, const RT *range, const int nc[3], const RenderArgs &a, unsigned char *out)
 {
     static_assert(IVR_MIP_LEVELS == 3, "k_cells reduces nodes of 8^3 cells");
-    const dim3 g((vol->nc[0] + 7) / 8, (vol->nc[1] + 7) / 8, (vol->nc[2] + 7) / 8);
-    hipLaunchKernelGGL(k_cells, g, dim3(512), 0, stream, vol->d_cellrange, out, vol->nc[0], vol->nc[1],
-                       vol->nc[2], a.lut, a.lutsize, a.lutscale, a.lutoffset, a.alpha);
+    const dim3 g((nc[0] + 7) / 8, (nc[1] + 7) / 8, (nc[2] + 7) / 8);
+    hipLaunchKernelGGL(k_cells<RT>, g, dim3(512), 0, stream, range, out, nc[0], nc[1], nc[2], a.lut, a.lutsize,
+                       a.lutscale, a.lutoffset, a.alpha);
     IVR_HIP(hipGetLastError());
     return IVR_OK;
+}
+
+} // namespace
+
+int ivr_launch_cells_on(hipStream_t stream, const ivr_volume *vol, const RenderArgs &a, unsigned char *out)
+{
+    return launch_cells(stream, vol->d_cellrange, vol->nc, a, out);
+}
+
+int ivr_launch_cells_half(hipStream_t stream, const __half2 *range, const int nc[3], const RenderArgs &a,
+                          unsigned char *out)
+{
+    return launch_cells(stream, range, nc, a, out);
 }
 
 int ivr_launch_composite(ivr_ctx *ctx, ivr_frame *fr, int ntiles)

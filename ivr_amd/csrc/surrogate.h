@@ -30,8 +30,8 @@
 
 /* macrocells of the surrogate renderer: 4^3 voxels (the plain marches use
  * 16^3), so that fewer samples that cannot contribute reach the network: a
- * network sample costs tens of times a plain one, the finer cells 9 bytes
- * per 64 voxels */
+ * network sample costs tens of times a plain one, the finer cells 5 bytes
+ * per 64 voxels (half-precision range, per-frame code) */
 #define SG_CELL_SHIFT 2
 
 /* bytes allocated past the half-precision tables: the encoding reads every
@@ -64,12 +64,12 @@ struct ivr_surrogate {
     unsigned char *d_fallback = nullptr;
     int step = 0;
     /* value range of every 4^3 macrocell (SG_CELL_SHIFT) of the trained
-     * volume (2 floats per 64 voxels): the surrogate skips cells that the
-     * frame's transfer function makes transparent, as the plain renderer
-     * does with its 16^3 cells */
+     * volume, as half (min, max) rounded outward (4 bytes per 64 voxels):
+     * the surrogate skips cells that the frame's transfer function makes
+     * transparent, as the plain renderer does with its 16^3 cells */
     int nc[3] = {0, 0, 0};
     size_t ncells = 0;
-    float2 *d_cellrange = nullptr;
+    __half2 *d_cellrange = nullptr;
     unsigned char *d_cellempty = nullptr;
 };
 

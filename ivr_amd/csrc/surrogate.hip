@@ -902,6 +902,29 @@ __global__ __launch_bounds__(256) void k_half(const float2 *src, __half2 *dst, s
         dst[i] = __float22half2_rn(src[i]);
 }
 
+/* the half at or below x (at or above with up): the nearest half, moved one
+ * step outward when rounding went the other way (its bit pattern is a
+ * sign-magnitude integer, so one step is +-1 on the pattern) */
+__device__ __forceinline__ __half half_outward(float x, bool up)
+{
+    const __half h = __float2half_rn(x);
+    const float back = __half2float(h);
+    if (up ? !(back < x) : !(back > x)) return h;
+    const unsigned short b = __half_as_ushort(h);
+    const bool neg = (b & 0x8000u) != 0;
+    unsigned short m;
+    if (up) m = neg ? (b == 0x8000u ? (unsigned short)0x0001u : (unsigned short)(b - 1u)) : (unsigned short)(b + 1u);
+    else m = neg ? (unsigned short)(b + 1u) : (b == 0u ? (unsigned short)0x8001u : (unsigned short)(b - 1u));
+    return __ushort_as_half(m);
+}
+
+/* cell ranges (min, max) -> half pairs that still contain them */
+__global__ __launch_bounds__(256) void k_range_half(const float2 *src, __half2 *dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = __halves2half2(half_outward(src[i].x, false), half_outward(src[i].y, true));
+}
+
 } // namespace
 
 /* ================================================================= host API */
@@ -1024,14 +1047,24 @@ int ivr_surrogate_create(ivr_ctx *ctx, const ivr_volume *vol, const ivr_surrogat
         (rc = ivr_dalloc(ctx, (void **)&s->frag, SG_NFRAG * 64 * 16)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->d_loss, 4)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->d_fallback, IVR_MAX_BRICKS)) ||
-        (rc = ivr_dalloc(ctx, (void **)&s->d_cellrange, sizeof(float2) * s->ncells)) ||
+        (rc = ivr_dalloc(ctx, (void **)&s->d_cellrange, sizeof(__half2) * s->ncells)) ||
         (rc = ivr_dalloc(ctx, (void **)&s->d_cellempty, s->ncells))) {
         ivr_surrogate_destroy(ctx, s);
         return rc;
     }
-    if ((rc = ivr_volume_cellrange(ctx, vol, SG_CELL_SHIFT, s->nc, s->d_cellrange))) {
-        ivr_surrogate_destroy(ctx, s);
-        return rc;
+    {
+        ivr_dscope range32(ctx);
+        if ((rc = range32.alloc(sizeof(float2) * s->ncells)) ||
+            (rc = ivr_volume_cellrange(ctx, vol, SG_CELL_SHIFT, s->nc, (float2 *)range32.p))) {
+            ivr_surrogate_destroy(ctx, s);
+            return rc;
+        }
+        hipLaunchKernelGGL(k_range_half, dim3((unsigned)std::min<size_t>((s->ncells + 255) / 256, 4096)), dim3(256),
+                           0, ctx->stream, (const float2 *)range32.p, s->d_cellrange, s->ncells);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            ivr_surrogate_destroy(ctx, s);
+            return ivr_fail(IVR_ERR_HIP, "macrocell ranges of the surrogate failed");
+        }
     }
     const unsigned seed = (unsigned)(sp->seed ^ (sp->seed >> 32));
     hipLaunchKernelGGL(k_init, dim3((unsigned)((s->ntable + 255) / 256)), dim3(256), 0, ctx->stream, s->tables,
@@ -1075,7 +1108,7 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s)
         ivr_dfree(ctx, s->frag, SG_NFRAG * 64 * 16);
         ivr_dfree(ctx, s->d_loss, 4);
         ivr_dfree(ctx, s->d_fallback, IVR_MAX_BRICKS);
-        ivr_dfree(ctx, s->d_cellrange, sizeof(float2) * s->ncells);
+        ivr_dfree(ctx, s->d_cellrange, sizeof(__half2) * s->ncells);
         ivr_dfree(ctx, s->d_cellempty, s->ncells);
         ivr_dfree(ctx, s->dE, 2 * s->dE_n * SG_IN * 4);
         train_streams_release(s);
@@ -1521,11 +1554,7 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
     ivr_set_grid(S.a, ctx, fr->W, fr->H, IVR_GRID_SURROGATE);
     const bool skip = d->skip && s->ncells > 0;
     if (skip) {
-        ivr_volume geo_cells; /* the surrogate's own macrocell ranges */
-        geo_cells.ncells = s->ncells;
-        for (int r = 0; r < 3; r++) geo_cells.nc[r] = s->nc[r];
-        geo_cells.d_cellrange = s->d_cellrange;
-        int rc = ivr_launch_cells_on(st, &geo_cells, S.a, cells);
+        int rc = ivr_launch_cells_half(st, s->d_cellrange, s->nc, S.a, cells); /* the surrogate's own cells */
         if (rc) return rc;
     }
     /* LDS: the LUT, the brick table (rounded to 16 bytes), the inference weights */
@@ -1734,10 +1763,10 @@ int ivr_surrogate_evaluate(ivr_ctx *ctx, ivr_surrogate *s, const ivr_volume *vol
 }
 
 /* checkpoint: "IVRSG005", params struct, dims, step, then MLP (params, m, v),
- * tables (values, m, v) and the 4^3-voxel macrocell ranges as little-endian
- * float32. Older files still load: "IVRSG004" and before hold ranges of
- * 16^3-voxel cells, which are read and dropped (the ranges computed from the
- * volume at create stay); "IVRSG003" and before stored a hashed level's
+ * tables (values, m, v) as little-endian float32, and the 4^3-voxel
+ * macrocell ranges as half pairs (4 bytes a cell). Older files still load:
+ * "IVRSG004" and before hold float32 ranges of 16^3-voxel cells, which are
+ * read and dropped (the ranges computed from the volume at create stay); "IVRSG003" and before stored a hashed level's
  * 4x4x4 block Morton-ordered in its slot (x0 y0 z0 x1 y1 z1), which is
  * permuted to the current order on load; "IVRSG002" files have a params
  * struct without log2_group (then 0), "IVRSG001" files no ranges either. */
@@ -1783,7 +1812,7 @@ int ivr_surrogate_save(ivr_ctx *ctx, const ivr_surrogate *s, const char *path)
               fwrite(hdr, sizeof hdr, 1, f) == 1;
     std::vector<float> buf;
     const float *dev[7] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt, (const float *)s->d_cellrange};
-    const size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, 2 * s->ncells};
+    const size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, s->ncells};
     for (int k = 0; k < 7 && ok; k++) {
         buf.resize(cnt[k]);
         if (hipMemcpy(buf.data(), dev[k], cnt[k] * 4, hipMemcpyDeviceToHost) != hipSuccess) ok = false;
@@ -1825,7 +1854,7 @@ int ivr_surrogate_load(ivr_ctx *ctx, ivr_surrogate *s, const char *path)
     /* every section is read into host memory before the device state is touched,
      * so a short or failed read leaves the surrogate as it was */
     float *dev[7] = {s->params, s->mp, s->vp, s->tables, s->mt, s->vt, (float *)s->d_cellrange};
-    size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, 2 * s->ncells};
+    size_t cnt[7] = {SG_NPARAM, SG_NPARAM, SG_NPARAM, s->ntable, s->ntable, s->ntable, s->ncells};
     /* files before "IVRSG005" hold ranges of 16^3-voxel cells */
     const bool old_ranges = nsec == 7 && magic[7] < '5';
     if (old_ranges) {

@@ -357,7 +357,7 @@ def _write_morton_checkpoint(src, dst, sg):
     ranges = off + 4 * (3 * nparam + 3 * ntab)
     n4 = int(np.prod([((int(n) - 2) >> 2) + 1 for n in hdr[2:5]]))
     n16 = int(np.prod([((int(n) - 2) >> 4) + 1 for n in hdr[2:5]]))
-    assert len(raw) == ranges + 4 * 2 * n4
+    assert len(raw) == ranges + 4 * n4  # a half (min, max) pair per 4^3 cell
     # ranges of 16^3 cells (the full range [0, 1] each): the loader reads and drops them
     raw = raw[:ranges] + np.tile(np.float32([0.0, 1.0]), n16).tobytes()
     raw[:8] = b"IVRSG003"
