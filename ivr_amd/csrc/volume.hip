@@ -253,7 +253,7 @@ __global__ void k_normalise(BrickGeom g, const float *src, VT *dst, size_t total
     dst[e] = (VT)gen_normalise(src[e], kind, lo, hi, scale);
 }
 
-/* per-cell min/max over the footprint voxels [16c, min(16c+16, n-1)] */
+/* per-cell min/max over the footprint voxels [c 2^shift, min((c+1) 2^shift, n-1)] per axis */
 template <typename VT>
 __global__ void k_cellrange(BrickGeom g, const unsigned long long *boff, const VT *src, int nc0,
                             int nc1, int nc2, int shift, float2 *out)
