@@ -427,6 +427,39 @@ def test_training_gain_fallback_and_checkpoint(ctx, small_vol):
 
 
 @pytest.mark.gpu
+def test_cell_ranges_contain_the_field(ctx):
+    """The surrogate's macrocell ranges are half pairs rounded outward.  A
+    field valued exactly f32(42/255) classifies into LUT entry 42 (visible),
+    while the nearest half, 0.16467..., falls into entry 41 (transparent):
+    ranges rounded to nearest would mark every cell of the block transparent
+    and skip it.  With every brick falling back, the surrogate's skipping
+    frame must equal the plain frame."""
+    v = np.float32(42) / np.float32(255)
+    assert int(v * np.float32(255)) == 42
+    assert int(np.float32(np.float16(v)) * np.float32(255)) == 41
+    host = np.zeros((64, 64, 64), np.float32)
+    host[16:48, 12:44, 20:52] = v
+    vol = ctx.volume(64, 64, 64, ivr.IVR_F32, brick=32)
+    vol.upload(host)
+    lut = np.zeros((256, 4), np.float32)
+    lut[42:] = (1.0, 0.5, 0.2, 0.3)
+    tf = ctx.tf(lut)
+    st = configs.state(2, 160, 144)
+    fr = ctx.frame(160, 144)
+    ctx.render(vol, tf, ivr.config_desc(st), fr)
+    plain = fr.download()
+    assert plain[..., 3].max() > 0  # the block is visible
+    sg = ctx.surrogate(vol, log2_table=12)
+    sg.set_fallback(np.ones(sg.nbricks, np.uint8))
+    sg.render(vol, tf, ivr.config_desc(st, skip=True), fr)
+    assert np.array_equal(fr.download(), plain)
+    sg.close()
+    fr.close()
+    tf.close()
+    vol.close()
+
+
+@pytest.mark.gpu
 def test_grouped_bricks_and_freeze(ctx, small_vol):
     """A surrogate whose bricks span 2x2x2 volume bricks (log2_group=1) over the
     64^3 volume in 32^3 bricks is the surrogate of the same volume in one 64^3
