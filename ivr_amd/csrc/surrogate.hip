@@ -64,6 +64,7 @@ __host__ __device__ __forceinline__ int kperm(int ks, int g, int j) { return (2 
 struct Enc {
     const __half2 *table16; /* [nbricks][L][T] feature pairs, the half-precision copy of the tables */
     unsigned T;
+    int log2T;
     int res[SG_L];
     unsigned n1[SG_L];  /* N+1 for dense levels, 0 for hashed */
 };
@@ -133,23 +134,45 @@ __device__ __forceinline__ float2 encode_level(const Enc &e, int brick, const Lv
     int v0[3];
     float w[3];
     level_cell(L, x, v0, w);
-    const size_t base = ((size_t)brick * SG_L + L.l) * (size_t)e.T;
-    const __half2 *tb = e.table16 + base;
+    const __half2 *tb = e.table16 + ((size_t)(brick * SG_L + L.l) << e.log2T);
+    /* The entries of corner rows (y + dy, z + dz), r = dy + 2 dz, as
+     * corner_index gives them, from per-level terms: dense levels
+     * x + n1 y + n1^2 z plus n1 dy + n1^2 dz (24-bit products: n1^3 <= T);
+     * hashed levels hash the block coordinates once, a row that crosses into
+     * the next block adding the axis prime, and place the corner in its slot
+     * from the low two bits of each coordinate.  Both are computed and one
+     * selected per row, with no multiply per corner. */
+    const unsigned X = (unsigned)v0[0], Y = (unsigned)v0[1], Z = (unsigned)v0[2];
+    const unsigned n1 = L.n1, n2 = __umul24(n1, n1);
+    const unsigned d0 = X + __umul24(n1, Y) + __umul24(n2, Z);
+    const unsigned mask = (e.T >> 6) - 1u, bx = X >> 2;
+    const unsigned hy0 = (Y >> 2) * 2654435761u, hz0 = (Z >> 2) * 805459861u;
+    const unsigned hy[2] = {hy0, (Y & 3u) == 3u ? hy0 + 2654435761u : hy0};
+    const unsigned hz[2] = {hz0, (Z & 3u) == 3u ? hz0 + 805459861u : hz0};
+    const unsigned ly[2] = {(Y & 1u) << 2 | (Y & 2u) << 3, ((Y + 1u) & 1u) << 2 | ((Y + 1u) & 2u) << 3};
+    const unsigned lz[2] = {(Z & 1u) << 3 | (Z & 2u) << 4, ((Z + 1u) & 1u) << 3 | ((Z + 1u) & 2u) << 4};
+    const bool dense = n1 != 0u;
     /* the corners (x, y, z) and (x+1, y, z) are consecutive entries on dense
      * levels (x < N) and on hashed levels unless x is the last of its block:
      * one 8-byte load per x-pair, and for that case a second load of the
-     * (x+1) corner into registers of its own (the half-precision copy has a
-     * 16-byte tail, so the pair load of the very last entry stays inside the
-     * allocation) */
-    const bool split = L.n1 == 0u && (v0[0] & 3) == 3;
+     * (x+1) corner, which lies at place 0 of the next block's slot, into
+     * registers of its own (the half-precision copy has a 16-byte tail, so
+     * the pair load of the very last entry stays inside the allocation) */
+    const bool split = !dense && (X & 3u) == 3u;
     unsigned pr[4][2], ex[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int r = 0; r < 4; r++)
-        __builtin_memcpy(pr[r], tb + corner_index(e.T, L, v0[0], v0[1] + (r & 1), v0[2] + (r >> 1)), 8);
+    for (int r = 0; r < 4; r++) {
+        const int dy = r & 1, dz = r >> 1;
+        const unsigned hashed = ((bx ^ hy[dy] ^ hz[dz]) & mask) << 6 | (X & 3u) | ly[dy] | lz[dz];
+        const unsigned dn = d0 + (dy ? n1 : 0u) + (dz ? n2 : 0u);
+        __builtin_memcpy(pr[r], tb + (dense ? dn : hashed), 8);
+    }
     if (split) {
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-            __builtin_memcpy(&ex[r], tb + corner_index(e.T, L, v0[0] + 1, v0[1] + (r & 1), v0[2] + (r >> 1)), 4);
+        for (int r = 0; r < 4; r++) {
+            const int dy = r & 1, dz = r >> 1;
+            __builtin_memcpy(&ex[r], tb + ((((bx + 1u) ^ hy[dy] ^ hz[dz]) & mask) << 6 | ly[dy] | lz[dz]), 4);
+        }
     }
     /* corners in the order c = dx + 2 dy + 4 dz, weights multiplied x, y, z */
     float2 f = make_float2(0.0f, 0.0f);
@@ -936,6 +959,7 @@ Enc make_enc(const ivr_surrogate *s)
     Enc e;
     e.table16 = (const __half2 *)s->tables16;
     e.T = s->T;
+    e.log2T = s->sp.log2_table;
     for (int l = 0; l < SG_L; l++) {
         e.res[l] = s->res[l];
         e.n1[l] = s->dense_n1[l];
