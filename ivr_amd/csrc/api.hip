@@ -112,6 +112,7 @@ int ivr_ctx_create(int device, ivr_ctx **out)
         c->block_w = (w == 16 || w == 32 || w == 64) ? w : 0;
     }
     if (const char *e = getenv("IVR_SL_SPLIT")) c->sl_split = atoi(e) != 0;
+    if (const char *e = getenv("IVR_SG_HELPERS")) c->sg_helpers = atoi(e) != 0;
     int rc = scratch_for(device, &c->scratch);
     if (rc) {
         delete c;

@@ -53,6 +53,7 @@ struct ivr_ctx {
     int wave_w = 0;
     int block_w = 0; /* pixel block of a workgroup: block_w x (256 / block_w) */
     int sl_split = 1; /* sort-last as one workgroup per (pixel block, run) */
+    int sg_helpers = 1; /* surrogate frames: lanes of ended rays evaluate live rays' next samples */
     /* this device's copy of the library's static scratch array (ivr_render_noalloc),
      * found by hipGetSymbolAddress when the context is created */
     unsigned char *scratch = nullptr;

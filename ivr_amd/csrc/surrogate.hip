@@ -1347,8 +1347,54 @@ struct SgRender {
     int only_brick;                /* >= 0: composite only that surrogate brick's samples */
     int snb[3], sbshift;           /* surrogate bricks per axis, log2 of their edge */
     int skip;                      /* macrocell skipping (k_render_sg_items) */
+    int helpers;                   /* lanes of ended rays evaluate live rays' next samples */
     int nblocks;                   /* pixel-block workgroups of the frame */
 };
+
+/* index of the r-th set bit (from 0) of m, which has more than r set bits */
+__device__ __forceinline__ int nth_bit(unsigned long long m, int r)
+{
+    int pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const unsigned long long low = m & ((1ull << w) - 1ull);
+        const int c = __popcll(low);
+        if (r >= c) {
+            r -= c;
+            m >>= w;
+            pos += w;
+        } else {
+            m = low;
+        }
+    }
+    return pos;
+}
+
+/* From sample index k of the ray (go, gd, tin, dt), the first index below
+ * kmax whose macrocell may hold a visible sample (kmax if none), with its
+ * position in p; the indices passed are added to skipped.  cc/e cache the
+ * code of the last cell looked up (a property of the cell, whatever the
+ * ray). */
+template <bool SKIP>
+__device__ __forceinline__ int sg_walk(const RenderArgs &a, const float *go, const float *gd, const float *rgd,
+                                       float tin, float dt, float rdt, int k, int kmax, int &cc, int &e, Pos &p,
+                                       unsigned &skipped)
+{
+    while (k < kmax) {
+        p = cfg_pos(a, go, gd, tin, dt, k);
+        if (!SKIP) break;
+        const int cell = cell_of<SG_CELL_SHIFT>(a, p);
+        if (cell != cc) {
+            cc = cell;
+            e = a.cellempty[cell];
+        }
+        if (!e) break;
+        const int kc = skip_empty<SG_CELL_SHIFT>(a, go, gd, rgd, tin, dt, rdt, p, e, k, kmax);
+        skipped += (unsigned)(kc - k);
+        k = kc;
+    }
+    return k;
+}
 
 /* Config-mode frame with the surrogate in place of the trilinear fetch.
  * The march is wave-synchronous: every lane evaluates one sample per
@@ -1359,7 +1405,15 @@ struct SgRender {
  * lanes with new pixels keeps all columns live but loses that coherence:
  * measured 2.1x slower on C4.)  With SKIP a lane first passes the samples
  * of transparent macrocells (the same exact jumps as the plain renderer),
- * so only samples that can contribute reach the network. */
+ * so only samples that can contribute reach the network.
+ *
+ * Helpers: at each step the r-th lane whose ray has ended takes the r-th
+ * live ray, walks it from where its owner starts to the sample after the
+ * owner's (the second visible one of the step) and evaluates that sample in
+ * the same network pass; the owner composites its own sample, then, unless
+ * the ray terminated, its helper's, with the indices between counted as
+ * passed.  Each sample's value depends on its position only, so the frame
+ * and its counts are those of one sample per ray and step, in fewer steps. */
 template <bool SKIP>
 __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float4 *smem)
 {
@@ -1400,27 +1454,55 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     int cc = -1, e = 0; /* macrocell of the transparency code held in e */
     Lvl lv[4];
     lane_levels(S.enc, lv);
+    const int lane = threadIdx.x & 63;
     while (__ballot(live) != 0ull) {
+        /* pair the r-th ended lane with the r-th live one (every lane of the
+         * wavefront runs this loop, so the ballots see all 64) */
+        const unsigned long long lm = __ballot(live);
+        bool helper = false;
+        int mate = lane; /* a helper's owner, an owner's helper, else the lane itself */
+        if (S.helpers) {
+            const unsigned long long dm = ~lm, below = (1ull << lane) - 1ull;
+            const int m = min(__popcll(lm), __popcll(dm));
+            if (live) {
+                const int r = __popcll(lm & below);
+                if (r < m) mate = nth_bit(dm, r);
+            } else {
+                const int r = __popcll(dm & below);
+                if (r < m) {
+                    mate = nth_bit(lm, r);
+                    helper = true;
+                }
+            }
+        }
+        /* the ray a lane walks: its own, or its owner's from where the owner starts */
+        const int src = helper ? mate : lane;
+        float wgo[3], wgd[3], wrgd[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            wgo[q] = __shfl(go[q], src, 64);
+            wgd[q] = __shfl(gd[q], src, 64);
+            if (SKIP) wrgd[q] = __shfl(rgd[q], src, 64);
+        }
+        const float wtin = __shfl(tin, src, 64), wdt = __shfl(dt, src, 64), wrdt = SKIP ? __shfl(rdt, src, 64) : 0.0f;
+        const int wn = __shfl(n, src, 64), wk = __shfl(k, src, 64);
+        Pos p;
+        int ks = wn;
+        if (live || helper) {
+            unsigned skipped = 0, passed = 0;
+            ks = sg_walk<SKIP>(a, wgo, wgd, wrgd, wtin, wdt, wrdt, wk, wn, cc, e, p, skipped);
+            if (helper) {
+                if (ks < wn) ks = sg_walk<SKIP>(a, wgo, wgd, wrgd, wtin, wdt, wrdt, ks + 1, wn, cc, e, p, passed);
+            } else {
+                cnt += skipped;
+                k = ks;
+                if (k >= n) live = false;
+            }
+        }
+        const bool sampled = live || (helper && ks < wn); /* the lane has a sample at p */
         float x[3] = {0.5f, 0.5f, 0.5f};
         int b = 0;
-        Pos p;
-        if (SKIP && live) {
-            while (k < n) {
-                p = cfg_pos(a, go, gd, tin, dt, k);
-                const int cell = cell_of<SG_CELL_SHIFT>(a, p);
-                if (cell != cc) {
-                    cc = cell;
-                    e = a.cellempty[cell];
-                }
-                if (!e) break;
-                const int kc = skip_empty<SG_CELL_SHIFT>(a, go, gd, rgd, tin, dt, rdt, p, e, k, n);
-                cnt += (unsigned)(kc - k);
-                k = kc;
-            }
-            if (k >= n) live = false;
-        }
-        if (live) {
-            if (!SKIP) p = cfg_pos(a, go, gd, tin, dt, k); /* with SKIP the walk above left it at k */
+        if (sampled) {
             const int sh = S.sbshift;
             const int bx = p.j >> sh, by = p.i >> sh, bz = p.k >> sh;
             b = bx + S.snb[0] * (by + S.snb[1] * bz);
@@ -1429,39 +1511,62 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
             x[2] = ((float)p.k + p.fz - (float)(bz << sh)) * invB;
         }
         /* The network runs only when a lane of the wavefront takes its value:
-         * wavefronts whose live samples all lie in fallback bricks (or outside
+         * wavefronts whose samples all lie in fallback bricks (or outside
          * only_brick) skip the encoding and the MLP.  Lanes that discard the value
          * evaluate the centre of their brick, which keeps their gathers off the
          * cache lines of samples that are not needed.  A lane's value depends on
          * its own sample only, so this changes no result. */
-        const bool net = live && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
+        const bool net = sampled && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
         if (!net) x[0] = x[1] = x[2] = 0.5f;
         float yv = 0.0f;
         if (__ballot(net) != 0ull) {
             yv = sg_infer(S.enc, lv, mlp, x, b);
             evals++;
         }
+        /* an owner's second sample: its helper's index (n if none) and value */
+        const int hl = live ? mate : lane;
+        const float y2 = __shfl(yv, hl, 64);
+        const int k2 = __shfl(ks, hl, 64);
         if (live) {
-            if (S.only_brick < 0 || S.only_brick == b) {
-                float s = yv;
-                if (S.fallback[b]) {
-                    float v[8];
-                    if (a.dtype_f16) fetch8<_Float16>(a, s_boff, p.j, p.i, p.k, v);
-                    else fetch8<float>(a, s_boff, p.j, p.i, p.k, v);
-                    s = trilinear(v, p.fx, p.fy, p.fz);
+            const bool two = hl != lane;
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                if (u == 1) {
+                    if (!live || !two) break;
+                    if (k2 >= n) { /* the helper's walk passed every remaining index */
+                        cnt += (unsigned)(n - k);
+                        k = n;
+                        live = false;
+                        break;
+                    }
+                    cnt += (unsigned)(k2 - k);
+                    k = k2;
+                    p = cfg_pos(a, go, gd, tin, dt, k);
+                    const int sh = S.sbshift;
+                    b = (p.j >> sh) + S.snb[0] * ((p.i >> sh) + S.snb[1] * (p.k >> sh));
+                    yv = y2;
                 }
-                fetched++; /* classified: by the network or the fallback */
-                const float4 e = s_lut[lut_index(a, s)];
-                const float al = e.w * a.alpha;
-                const float w = (1.0f - acc.a) * al;
-                acc.r = fmaf(w, e.x, acc.r);
-                acc.g = fmaf(w, e.y, acc.g);
-                acc.b = fmaf(w, e.z, acc.b);
-                acc.a = acc.a + w;
-                cnt++;
+                if (S.only_brick < 0 || S.only_brick == b) {
+                    float s = yv;
+                    if (S.fallback[b]) {
+                        float v[8];
+                        if (a.dtype_f16) fetch8<_Float16>(a, s_boff, p.j, p.i, p.k, v);
+                        else fetch8<float>(a, s_boff, p.j, p.i, p.k, v);
+                        s = trilinear(v, p.fx, p.fy, p.fz);
+                    }
+                    fetched++; /* classified: by the network or the fallback */
+                    const float4 e = s_lut[lut_index(a, s)];
+                    const float al = e.w * a.alpha;
+                    const float w = (1.0f - acc.a) * al;
+                    acc.r = fmaf(w, e.x, acc.r);
+                    acc.g = fmaf(w, e.y, acc.g);
+                    acc.b = fmaf(w, e.z, acc.b);
+                    acc.a = acc.a + w;
+                    cnt++;
+                }
+                k++;
+                if (acc.a >= a.ert || k >= n) live = false;
             }
-            k++;
-            if (acc.a >= a.ert || k >= n) live = false;
         }
     }
     if (inside) {
@@ -1587,6 +1692,7 @@ int sg_prepare(ivr_ctx *ctx, hipStream_t st, const ivr_surrogate *s, const ivr_v
         return ivr_fail(IVR_ERR_UNSUPPORTED, "LUT of %d entries, %d bricks and the network exceed 64 KiB of LDS",
                         S.a.lutsize, S.a.nbricks);
     S.skip = skip ? 1 : 0;
+    S.helpers = ctx->sg_helpers;
     S.nblocks = S.a.nblk_pad;
     return IVR_OK;
 }

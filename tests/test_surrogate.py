@@ -460,6 +460,56 @@ def test_cell_ranges_contain_the_field(ctx):
 
 
 @pytest.mark.gpu
+def test_helper_lanes_change_nothing(ctx, small_vol):
+    """Lanes whose ray has ended evaluate live rays' next samples: the same
+    trained network, loaded into a context made with IVR_SG_HELPERS=0, gives
+    the same frames, sample counts and per-brick evaluation, with and
+    without skipping and with some bricks falling back."""
+    vol, host = small_vol
+    lut = configs.tf_lut(2)
+    st = configs.state(2, 160, 144)
+    tf = ctx.tf(lut)
+    fr = ctx.frame(160, 144)
+    sg = ctx.surrogate(vol, log2_table=15)
+    for _ in range(3):
+        sg.train(vol, 20, 1 << 18)
+    path = os.path.join(tempfile.mkdtemp(), "sg.ckpt")
+    sg.save(path)
+    os.environ["IVR_SG_HELPERS"] = "0"
+    try:
+        ctx2 = ivr.Context(0)
+    finally:
+        del os.environ["IVR_SG_HELPERS"]
+    vol2 = ctx2.volume(64, 64, 64, ivr.IVR_F32, brick=32)
+    vol2.upload(host)
+    tf2 = ctx2.tf(lut)
+    fr2 = ctx2.frame(160, 144)
+    sg2 = ctx2.surrogate(vol2, log2_table=15)
+    sg2.load(path)
+    fallback = (np.arange(sg.nbricks) % 3 == 1).astype(np.uint8)
+    for flags in (np.zeros(sg.nbricks, np.uint8), fallback):
+        sg.set_fallback(flags)
+        sg2.set_fallback(flags)
+        for skip in (False, True):
+            d = ivr.config_desc(st, skip=skip)
+            sg.render(vol, tf, d, fr)
+            sg2.render(vol2, tf2, d, fr2)
+            a, b = fr.download(), fr2.download()
+            assert a[..., 3].max() > 0
+            assert np.array_equal(a, b), (flags.tolist(), skip)
+            sa, sb = fr.stats(), fr2.stats()
+            assert (sa["samples"], sa["visited"]) == (sb["samples"], sb["visited"])
+            assert sa["evaluations"] <= sb["evaluations"]  # helpers fill lane slots
+    d = ivr.config_desc(st, skip=True)
+    pa, na = sg.evaluate(vol, tf, d, threshold_db=40.0)
+    pb, nb = sg2.evaluate(vol2, tf2, d, threshold_db=40.0)
+    assert na == nb and np.array_equal(pa, pb)
+    for o in (sg2, fr2, tf2, vol2, sg, fr, tf):
+        o.close()
+    ctx2.close()
+
+
+@pytest.mark.gpu
 def test_grouped_bricks_and_freeze(ctx, small_vol):
     """A surrogate whose bricks span 2x2x2 volume bricks (log2_group=1) over the
     64^3 volume in 32^3 bricks is the surrogate of the same volume in one 64^3
