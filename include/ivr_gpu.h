@@ -233,7 +233,7 @@ int ivr_surrogate_destroy(ivr_ctx *ctx, ivr_surrogate *s);
 /* `steps` Adam steps of `samples` samples whose targets are the trilinear
  * values of `vol`; loss_out = last MSE.  The samples of a step go to the
  * bricks in equal contiguous ranges, each uniform in its brick (stratified
- * in tiles of 64), and are processed in chunks of up to 2^21 on the
+ * in tiles of 64), and are processed in chunks of up to 2^22 on the
  * context's stream and one more stream of the surrogate's own; the first
  * call allocates two chunk buffers of encoding gradients (128 B per sample
  * of a chunk each), which ivr_surrogate_freeze releases. */
