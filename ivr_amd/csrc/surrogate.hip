@@ -1456,6 +1456,12 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
     Lvl lv[4];
     lane_levels(S.enc, lv);
     const int lane = threadIdx.x & 63;
+    /* the fallback flags as a mask when the surrogate has at most 64 bricks,
+     * so the march does not wait on a load for them at every step */
+    const int nsb = S.snb[0] * S.snb[1] * S.snb[2];
+    const bool fbmask = nsb <= 64;
+    const unsigned long long fbm = fbmask ? __ballot(lane < nsb && S.fallback[lane] != 0) : 0ull;
+    auto falls_back = [&](int bb) { return fbmask ? ((fbm >> bb) & 1ull) != 0ull : S.fallback[bb] != 0; };
     while (__ballot(live) != 0ull) {
         /* pair the r-th ended lane with the r-th live one (every lane of the
          * wavefront runs this loop, so the ballots see all 64) */
@@ -1517,7 +1523,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
          * evaluate the centre of their brick, which keeps their gathers off the
          * cache lines of samples that are not needed.  A lane's value depends on
          * its own sample only, so this changes no result. */
-        const bool net = sampled && !S.fallback[b] && (S.only_brick < 0 || S.only_brick == b);
+        const bool net = sampled && !falls_back(b) && (S.only_brick < 0 || S.only_brick == b);
         if (!net) x[0] = x[1] = x[2] = 0.5f;
         float yv = 0.0f;
         if (__ballot(net) != 0ull) {
@@ -1549,7 +1555,7 @@ __device__ __forceinline__ void sg_frame_block(const SgRender &S, int blk, float
                 }
                 if (S.only_brick < 0 || S.only_brick == b) {
                     float s = yv;
-                    if (S.fallback[b]) {
+                    if (falls_back(b)) {
                         float v[8];
                         if (a.dtype_f16) fetch8<_Float16>(a, s_boff, p.j, p.i, p.k, v);
                         else fetch8<float>(a, s_boff, p.j, p.i, p.k, v);
