@@ -4,6 +4,9 @@ x 2 features, MLP 32-64-64-1 on bf16 MFMA) trained by the project's kernels
 and rendered at 1920x1080, and the steering service with the 256^3 fp32,
 512^3 fp32 and 1024^3 fp16 volumes resident and requests at every
 resolution of the mix."""
+import os
+import tempfile
+
 import numpy as np
 import pytest
 
@@ -28,7 +31,9 @@ def test_surrogate_of_the_512_volume_at_1080p(ctx, c2_volume):
     least 40 dB from the plain GPU render; the per-brick flags are exactly
     the bricks below 40 dB; a frame with every brick falling back is the
     plain frame bit for bit; the renderer's network value at points of two
-    bricks matches the oracle on the trained, fp16-rounded tables"""
+    bricks matches the oracle on the trained, fp16-rounded tables; the same
+    network in a context without helper lanes (IVR_SG_HELPERS=0) renders
+    the same frame and counts"""
     vol = c2_volume
     sg = ctx.surrogate(vol)
     assert sg.nbricks == 8 and sg.T == 1 << 19
@@ -70,6 +75,34 @@ def test_surrogate_of_the_512_volume_at_1080p(ctx, c2_volume):
     err = np.abs(y - yo).max()
     print("trained network: max |y_render - y_oracle| = %.3e" % err)
     assert err < 2e-3 * max(1.0, np.abs(yo).max())
+    # helper lanes change no frame: the same checkpoint without them
+    sg.set_fallback(np.zeros(sg.nbricks, np.uint8))
+    sg.render(vol, tf, desc, fr)
+    img, st = fr.download(), fr.stats()
+    path = os.path.join(tempfile.mkdtemp(), "c4.ckpt")
+    sg.save(path)
+    os.environ["IVR_SG_HELPERS"] = "0"
+    try:
+        ctx2 = ivr.Context(0)
+    finally:
+        del os.environ["IVR_SG_HELPERS"]
+    cfg = configs.WORKLOADS[2]
+    vol2 = ctx2.volume(*cfg["dims"], cfg["dtype"], brick=cfg["brick"])
+    vol2.generate(configs.gen_params(2))
+    sg2 = ctx2.surrogate(vol2)
+    sg2.load(path)
+    sg2.set_fallback(np.zeros(sg2.nbricks, np.uint8))
+    tf2 = ctx2.tf(configs.tf_lut(2))
+    fr2 = ctx2.frame(W, H)
+    sg2.render(vol2, tf2, desc, fr2)
+    st2 = fr2.stats()
+    print("lane utilisation with helpers %.3f, without %.3f"
+          % (st["visited"] / st["evaluations"], st2["visited"] / st2["evaluations"]))
+    assert np.array_equal(fr2.download(), img)
+    assert (st["samples"], st["visited"]) == (st2["samples"], st2["visited"])
+    for o in (fr2, tf2, sg2, vol2):
+        o.close()
+    ctx2.close()
     fr.close()
     tf.close()
     sg.close()
