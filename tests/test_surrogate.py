@@ -460,6 +460,40 @@ def test_cell_ranges_contain_the_field(ctx):
 
 
 @pytest.mark.gpu
+def test_gradient_over_several_chunks(ctx, small_vol):
+    """Training runs in chunks of 2^22 samples (network pass, then the table
+    scatter on a second stream, encoding gradients double-buffered).  Over
+    2^22 + 2^20 + 37 samples, two chunks and the second ragged, the network
+    outputs equal those of the two parts computed alone, bit for bit, and
+    the mean gradients equal the parts' gradients weighted by their sample
+    counts, up to the rounding of float atomics summed in another order."""
+    vol, _ = small_vol
+    sg = ctx.surrogate(vol, log2_table=15, seed=5)
+    rng = np.random.default_rng(12)
+    n1, n2 = 1 << 22, (1 << 20) + 37
+    n = n1 + n2
+    xyz = rng.uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    bricks = rng.integers(0, sg.nbricks, n).astype(np.int32)
+    t = rng.uniform(0.0, 1.0, n).astype(np.float32)
+    y, loss, g, gt = sg.grad(xyz, bricks, t, table_brick=3)
+    y1, l1, g1, gt1 = sg.grad(xyz[:n1], bricks[:n1], t[:n1], table_brick=3)
+    y2, l2, g2, gt2 = sg.grad(xyz[n1:], bricks[n1:], t[n1:], table_brick=3)
+    assert np.array_equal(y, np.concatenate([y1, y2]))
+    assert abs(loss - (n1 * l1 + n2 * l2) / n) <= 1e-5 * loss
+    gw = (n1 * g1.astype(np.float64) + n2 * g2) / n
+    assert np.abs(g - gw).max() <= 1e-4 * np.abs(gw).max()
+    # a coarse vertex sums about a thousand per-sample terms of ~1e-7 that
+    # largely cancel, so fp32 rounding of the partial sums reaches ~1e-3 of the
+    # largest entry; a chunk lost or added, or read from the wrong buffer,
+    # moves entries by tens of percent
+    tw = (n1 * gt1.astype(np.float64) + n2 * gt2) / n
+    assert np.abs(gt - tw).max() <= 1e-2 * np.abs(tw).max()
+    assert np.linalg.norm(gt - tw) <= 1e-2 * np.linalg.norm(tw)
+    assert np.count_nonzero(gt) > 0
+    sg.close()
+
+
+@pytest.mark.gpu
 def test_helper_lanes_change_nothing(ctx, small_vol):
     """Lanes whose ray has ended evaluate live rays' next samples: the same
     trained network, loaded into a context made with IVR_SG_HELPERS=0, gives
