@@ -229,10 +229,13 @@ def main():
             "frac": round(achieved / HBM_PEAK, 4),
             "traffic": traffic,
             "traffic_frac": round(traffic / (k_ms * 1e-3) / HBM_PEAK, 4) if traffic else None,
+            "compulsory_bytes": compulsory_bytes(cfg, W, H),
+            "l2_hit": measured_l2_hit(cfg["name"], comp_name, skip),
             "note": "achieved counts 8 voxels per classified sample with no reuse (plus the partial "
                     "planes); neighbouring samples and rays share footprint voxels in L1/L2, so this "
                     "rate can pass the HBM peak; traffic is the measured HBM bytes per launch and "
-                    "traffic_frac their rate against the peak",
+                    "traffic_frac their rate against the peak; compulsory_bytes is the floor of "
+                    "ghosted brick bytes touched plus the image; l2_hit from the committed counters",
         },
         "cpu_baseline": cpu,
         "reference_cpu": ref,
@@ -313,7 +316,9 @@ def c3_measure(ctx, vol, steps):
                          "combine_ms": round(c_ms, 4), "algorithmic_bytes": bytes_alg,
                          "achieved": round(bytes_alg / (k_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK / 1e9,
                          "unit": "GB/s", "frac": round(bytes_alg / (k_ms * 1e-3) / HBM_PEAK, 4),
-                         "traffic": measured_traffic(cfg["name"], "sort_last", True)},
+                         "traffic": measured_traffic(cfg["name"], "sort_last", True),
+                         "compulsory_bytes": compulsory_bytes(cfg, W, H),
+                         "l2_hit": measured_l2_hit(cfg["name"], "sort_last", True)},
             "max_abs_diff_vs_oracle": dmax, "oracle_rows": [r0, r1]}
 
 
@@ -719,15 +724,40 @@ def bench_gradient(args):
     ctx.close()
 
 
-def measured_traffic(workload, comp_name, skip):
-    """HBM bytes per launch from the committed PMC summary, or None"""
+def _traffic_entry(workload, comp_name, skip):
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     for e in json.load(open(path)).get("entries", []):
         if e["workload"] == workload and e["composite"] == comp_name and e["skip"] == skip:
-            return int(e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024)
+            return e
     return None
+
+
+def measured_traffic(workload, comp_name, skip):
+    """HBM bytes per launch from the committed PMC summary, or None"""
+    e = _traffic_entry(workload, comp_name, skip)
+    if e is None:
+        return None
+    return int(e["fetch_kib"] * e["fetch_scale"] * 1024 + e["write_kib"] * e["write_scale"] * 1024)
+
+
+def measured_l2_hit(workload, comp_name, skip):
+    """L2 hit rate of the kernel from the committed PMC summary, or None"""
+    e = _traffic_entry(workload, comp_name, skip)
+    return e.get("l2_hit") if e else None
+
+
+def compulsory_bytes(cfg, W, H):
+    """the compulsory-traffic floor of a frame: the ghosted bytes of the bricks
+    it touches (the orbit cameras keep the whole volume in view, so every
+    brick) plus the RGBA8 image"""
+    import ivr_amd as ivr
+    nb = 1
+    for d in cfg["dims"]:
+        nb *= -(-d // cfg["brick"])
+    esz = 4 if cfg["dtype"] == ivr.IVR_F32 else 2
+    return nb * (cfg["brick"] + 2) ** 3 * esz + 4 * W * H
 
 
 def copy_calibration(mib):
